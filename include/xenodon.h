@@ -1,0 +1,163 @@
+/*
+ * xenodon_amd C-ABI: MI355X (gfx950) volume ray traversal.
+ *
+ * Drop-in boundary for Xenodon's Vulkan compute dispatch and headless
+ * backend.  Plain pointers and sizes only; every call returns an xd_status
+ * (0 = XD_OK) and leaves a thread-local message for xd_last_error().  The C++
+ * host layer rethrows it as the reference's Error so CLI text is unchanged
+ * (src/core/Error.h:8-13, src/main.cpp:161-195).
+ *
+ * Threading: a context owns one HIP stream on one device and may be driven
+ * by one host thread at a time (the reference renders from one thread and
+ * waits on a fence per frame, src/backend/headless/HeadlessOutput.cpp:90-93).
+ */
+#ifndef XENODON_AMD_H
+#define XENODON_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define XD_API __attribute__((visibility("default")))
+#else
+#define XD_API
+#endif
+
+typedef enum {
+    XD_OK = 0,
+    XD_ERR_INVALID_ARGUMENT = 1,
+    XD_ERR_DEVICE = 2,
+    XD_ERR_OUT_OF_MEMORY = 3,
+    XD_ERR_INCOMPATIBLE = 4, /* traversal vs volume kind, main_loop.cpp:109-126 */
+    XD_ERR_FORMAT = 5,       /* malformed volume data */
+    XD_ERR_IO = 6,
+    XD_ERR_REJECTED = 7,     /* admission refused (serving) */
+    XD_ERR_INTERNAL = 8
+} xd_status;
+
+/* Traversal ids in the order of SHADER_OPTIONS (src/main_loop.cpp:37-43). */
+typedef enum {
+    XD_DDA = 0,       /* resources/dda.comp        requires a grid volume */
+    XD_SVO_NAIVE = 1, /* resources/svo_naive.comp  requires an SVO volume */
+    XD_ESVO = 2,      /* resources/esvo.comp       requires an SVO volume */
+    XD_SVO_DF = 3,    /* resources/svo_df.comp     requires an SVO volume */
+    XD_SVO_ROPE = 4   /* resources/svo_rope.comp   SVO made by `convert --rope`;
+                         on other trees it integrates the first leaf only,
+                         exactly as the reference shader does */
+} xd_traversal;
+
+typedef enum { XD_VOLUME_GRID = 0, XD_VOLUME_SVO = 1 } xd_volume_kind;
+
+/* Push constant Camera, 48 B (resources/common.glsl:6-10; host fill
+ * src/render/Renderer.cpp:58-64).  translation is in unit-cube space already
+ * divided by voxel_ratio; the fourth components are ignored. */
+typedef struct {
+    float forward[4];
+    float up[4];
+    float translation[4];
+} xd_camera;
+
+/* vk::Rect2D */
+typedef struct {
+    int32_t offset[2];
+    uint32_t extent[2];
+} xd_rect;
+
+/* Uniform buffer (resources/common.glsl:12-33; host fill
+ * src/render/Renderer.cpp:236-268, src/main_loop.cpp:197-201).
+ * model_dim = grid dimensions, or the SVO side on all three axes. */
+typedef struct {
+    xd_rect output_region;
+    xd_rect display_region;
+    float voxel_ratio[4];
+    uint32_t model_dim[4];
+    float emission_coeff;
+} xd_uniforms;
+
+typedef struct xd_context xd_context;
+typedef struct xd_volume xd_volume;
+typedef struct xd_frame xd_frame;
+
+typedef struct {
+    int32_t kind;          /* xd_volume_kind */
+    uint32_t max_depth;    /* SVO: deepest reachable node; grid: 0 */
+    uint64_t dims[3];      /* grid dimensions, or side x3 for an SVO */
+    uint64_t side;         /* SVO side (power of two); grid: max(dims) */
+    uint64_t node_count;   /* SVO nodes; grid: 0 */
+    uint64_t device_bytes; /* HBM held by this volume */
+    uint32_t is_rope;      /* leaves carry ropes (all leaf children[6..7]==0 and
+                              some rope != 0) */
+    uint32_t reserved;
+} xd_volume_info;
+
+/* ---- errors / devices --------------------------------------------------- */
+
+/* Message of the last failing call on this thread ("" if none). */
+XD_API const char* xd_last_error(void);
+XD_API int xd_device_count(int* count);
+/* Name and PCI id of a device (xenodon sysinfo, src/sysinfo.cpp:8-80). */
+XD_API int xd_device_describe(int device_index, char* name, size_t name_len, uint32_t* pci_bus, uint32_t* pci_device,
+                              uint64_t* total_bytes);
+
+/* ---- context (replaces Device/Queue/CommandPool, src/graphics/) ---------- */
+
+/* device_index is the headless config's vkindex
+ * (src/backend/headless/HeadlessConfig.cpp:5-16). */
+XD_API int xd_open(int device_index, xd_context** out);
+/* Waits for the context's work, then frees the context together with every
+ * volume and frame still alive on it; their handles become invalid. */
+XD_API int xd_close(xd_context* ctx);
+
+/* ---- volumes (replaces Dda/SvoRaytraceResources, src/render/) ------------ */
+
+/* Copies an RGBA8 grid (x fastest, then y, then z; src/model/Grid.h:50-56)
+ * into one HBM allocation (no 4 GiB cap).  Replaces
+ * DdaRaytraceResources::DdaRaytraceResources (DdaRaytraceAlgorithm.cpp:15-97). */
+XD_API int xd_volume_from_grid(xd_context* ctx, const void* rgba8, uint64_t dim_x, uint64_t dim_y, uint64_t dim_z,
+                               xd_volume** out);
+/* Copies node_count nodes of 10 little-endian u32 words each (children[8],
+ * color, is_leaf_depth; src/model/Octree.h:35-45) into one HBM allocation.
+ * Rejects trees with cycles or out-of-range children (XD_ERR_FORMAT).
+ * Replaces SvoRaytraceResources::SvoRaytraceResources
+ * (SvoRaytraceAlgorithm.cpp:12-48). */
+XD_API int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, uint64_t side,
+                              xd_volume** out);
+XD_API int xd_volume_info_get(const xd_volume* vol, xd_volume_info* out);
+XD_API int xd_volume_free(xd_volume* vol);
+
+/* ---- frames (replaces HeadlessOutput's render target) -------------------- */
+
+XD_API int xd_frame_create(xd_context* ctx, uint32_t width, uint32_t height, xd_frame** out);
+XD_API int xd_frame_free(xd_frame* frame);
+/* Device address of the frame's RGBA8 pixels (row pitch = width). */
+XD_API int xd_frame_device_ptr(xd_frame* frame, void** out);
+
+/* ---- plain render: one launch per output per frame ----------------------- */
+
+/* Renders output_region of the display into frame (whose size must be the
+ * region's extent), waits for completion and returns the launch's own
+ * duration in ms measured with HIP events around the launch, the semantics
+ * of RenderStatsCollector (src/render/RenderStats.cpp:46-83).  Replaces
+ * Renderer::render (src/render/Renderer.cpp:55-103) + the fence wait. */
+XD_API int xd_render(xd_context* ctx, const xd_volume* vol, int traversal, const xd_camera* cam,
+                     const xd_uniforms* uniforms, xd_frame* frame, double* dispatch_ms);
+
+/* Strided copy of the frame into host memory (stride in pixels, 0 = width),
+ * replacing HeadlessOutput::download (HeadlessOutput.cpp:95-136). */
+XD_API int xd_frame_download(xd_frame* frame, uint32_t* dst, uint64_t stride_px);
+
+/* Number of 32-bit volume words the reference algorithm reads for this
+ * frame (the roofline's algorithmic bytes = 4*words + 4*pixels).  Runs an
+ * instrumented variant of the kernel with every shortcut disabled. */
+XD_API int xd_count_words(xd_context* ctx, const xd_volume* vol, int traversal, const xd_camera* cam,
+                          const xd_uniforms* uniforms, uint64_t* words);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,264 @@
+"""ctypes binding of the C-ABI in include/xenodon.h (libxenodon_amd.so).
+
+This is plumbing only: every call goes straight to the native library, and
+importing fails loudly when the library has not been built -- there is no
+CPU fallback anywhere in the package.
+"""
+import ctypes
+import os
+import weakref
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libxenodon_amd.so")
+
+XD_OK = 0
+DDA, SVO_NAIVE, ESVO, SVO_DF, SVO_ROPE = range(5)
+# src/main_loop.cpp:37-43 option names
+TRAVERSALS = {"dda": DDA, "svo-naive": SVO_NAIVE, "esvo": ESVO, "svo-df": SVO_DF, "svo-rope": SVO_ROPE}
+TRAVERSAL_NAMES = {v: k for k, v in TRAVERSALS.items()}
+VOLUME_GRID, VOLUME_SVO = 0, 1
+
+
+class XdError(RuntimeError):
+    """Error raised by a C-ABI call; .status is the xd_status code."""
+
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("forward", ctypes.c_float * 4), ("up", ctypes.c_float * 4), ("translation", ctypes.c_float * 4)]
+
+
+class Rect(ctypes.Structure):
+    _fields_ = [("offset", ctypes.c_int32 * 2), ("extent", ctypes.c_uint32 * 2)]
+
+
+class Uniforms(ctypes.Structure):
+    _fields_ = [
+        ("output_region", Rect),
+        ("display_region", Rect),
+        ("voxel_ratio", ctypes.c_float * 4),
+        ("model_dim", ctypes.c_uint32 * 4),
+        ("emission_coeff", ctypes.c_float),
+    ]
+
+
+class VolumeInfo(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("max_depth", ctypes.c_uint32),
+        ("dims", ctypes.c_uint64 * 3),
+        ("side", ctypes.c_uint64),
+        ("node_count", ctypes.c_uint64),
+        ("device_bytes", ctypes.c_uint64),
+        ("is_rope", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def _declare(L):
+    P = ctypes.POINTER
+    vp = ctypes.c_void_p
+    sigs = {
+        "xd_last_error": ([], ctypes.c_char_p),
+        "xd_device_count": ([P(ctypes.c_int)], ctypes.c_int),
+        "xd_device_describe": ([ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, P(ctypes.c_uint32),
+                                P(ctypes.c_uint32), P(ctypes.c_uint64)], ctypes.c_int),
+        "xd_open": ([ctypes.c_int, P(vp)], ctypes.c_int),
+        "xd_close": ([vp], ctypes.c_int),
+        "xd_volume_from_grid": ([vp, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, P(vp)], ctypes.c_int),
+        "xd_volume_from_svo": ([vp, vp, ctypes.c_uint64, ctypes.c_uint64, P(vp)], ctypes.c_int),
+        "xd_volume_info_get": ([vp, P(VolumeInfo)], ctypes.c_int),
+        "xd_volume_free": ([vp], ctypes.c_int),
+        "xd_frame_create": ([vp, ctypes.c_uint32, ctypes.c_uint32, P(vp)], ctypes.c_int),
+        "xd_frame_free": ([vp], ctypes.c_int),
+        "xd_frame_device_ptr": ([vp, P(vp)], ctypes.c_int),
+        "xd_render": ([vp, vp, ctypes.c_int, P(Camera), P(Uniforms), vp, P(ctypes.c_double)], ctypes.c_int),
+        "xd_frame_download": ([vp, vp, ctypes.c_uint64], ctypes.c_int),
+        "xd_count_words": ([vp, vp, ctypes.c_int, P(Camera), P(Uniforms), P(ctypes.c_uint64)], ctypes.c_int),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    return L
+
+
+def lib():
+    """The loaded native library.  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m xenodon_amd.build` "
+                              "(the HIP path has no fallback)")
+        _lib = _declare(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def check(status):
+    if status != XD_OK:
+        msg = lib().xd_last_error()
+        raise XdError(status, msg.decode() if msg else f"xd status {status}")
+
+
+def device_count():
+    n = ctypes.c_int()
+    st = lib().xd_device_count(ctypes.byref(n))
+    return n.value if st == XD_OK else 0
+
+
+def make_camera(forward, up, translation):
+    c = Camera()
+    c.forward[:] = [float(v) for v in forward[:3]] + [0.0]
+    c.up[:] = [float(v) for v in up[:3]] + [0.0]
+    c.translation[:] = [float(v) for v in translation[:3]] + [0.0]
+    return c
+
+
+def make_uniforms(width, height, model_dim, emission=1.0, voxel_ratio=(1.0, 1.0, 1.0), out_offset=(0, 0),
+                  disp_offset=(0, 0), disp_extent=None):
+    u = Uniforms()
+    u.output_region.offset[:] = list(out_offset)
+    u.output_region.extent[:] = [width, height]
+    u.display_region.offset[:] = list(disp_offset)
+    u.display_region.extent[:] = list(disp_extent) if disp_extent is not None else [width, height]
+    u.voxel_ratio[:] = [float(v) for v in voxel_ratio] + [0.0]
+    u.model_dim[:] = [int(v) for v in model_dim] + [0]
+    u.emission_coeff = float(emission)
+    return u
+
+
+class Context:
+    """One device + one HIP stream (xd_open)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().xd_open(device, ctypes.byref(h)))
+        self.h = h
+        self._children = weakref.WeakSet()
+
+    def close(self):
+        if self.h:
+            # xd_close frees the context's remaining volumes and frames
+            for c in list(self._children):
+                c.h = None
+            lib().xd_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grid_volume(self, grid):
+        """grid: uint8 array [Z, Y, X, 4] (x fastest)."""
+        g = np.ascontiguousarray(grid, dtype=np.uint8)
+        if g.ndim != 4 or g.shape[3] != 4:
+            raise ValueError("grid must be [Z, Y, X, 4] uint8")
+        h = ctypes.c_void_p()
+        check(lib().xd_volume_from_grid(self.h, g.ctypes.data, g.shape[2], g.shape[1], g.shape[0], ctypes.byref(h)))
+        return self._adopt(Volume(self, h))
+
+    def svo_volume(self, nodes, side):
+        """nodes: uint32 [N, 10] (children[8], color, is_leaf_depth)."""
+        n = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 10)
+        h = ctypes.c_void_p()
+        check(lib().xd_volume_from_svo(self.h, n.ctypes.data, n.shape[0], int(side), ctypes.byref(h)))
+        return self._adopt(Volume(self, h))
+
+    def frame(self, width, height):
+        h = ctypes.c_void_p()
+        check(lib().xd_frame_create(self.h, width, height, ctypes.byref(h)))
+        return self._adopt(Frame(self, h, width, height))
+
+    def _adopt(self, child):
+        self._children.add(child)
+        return child
+
+    def render(self, volume, traversal, cam, uni, frame):
+        """Plain one-launch render; returns the launch time in ms."""
+        ms = ctypes.c_double()
+        check(lib().xd_render(self.h, volume.h, traversal, ctypes.byref(cam), ctypes.byref(uni), frame.h,
+                              ctypes.byref(ms)))
+        return ms.value
+
+    def count_words(self, volume, traversal, cam, uni):
+        w = ctypes.c_uint64()
+        check(lib().xd_count_words(self.h, volume.h, traversal, ctypes.byref(cam), ctypes.byref(uni),
+                                   ctypes.byref(w)))
+        return w.value
+
+    def render_image(self, volume, traversal, cam, uni):
+        """Render into a fresh frame and return ([H, W, 4] uint8, ms)."""
+        W, H = uni.output_region.extent[0], uni.output_region.extent[1]
+        f = self.frame(W, H)
+        try:
+            ms = self.render(volume, traversal, cam, uni, f)
+            return f.download(), ms
+        finally:
+            f.free()
+
+
+class Volume:
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+
+    def info(self):
+        i = VolumeInfo()
+        check(lib().xd_volume_info_get(self.h, ctypes.byref(i)))
+        return i
+
+    def free(self):
+        if self.h:
+            lib().xd_volume_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Frame:
+    def __init__(self, ctx, h, w, hgt):
+        self.ctx = ctx
+        self.h = h
+        self.width = w
+        self.height = hgt
+
+    def download(self):
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        check(lib().xd_frame_download(self.h, out.ctypes.data, 0))
+        return out
+
+    def device_ptr(self):
+        p = ctypes.c_void_p()
+        check(lib().xd_frame_device_ptr(self.h, ctypes.byref(p)))
+        return p.value
+
+    def free(self):
+        if self.h:
+            lib().xd_frame_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

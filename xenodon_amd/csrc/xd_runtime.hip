@@ -1,0 +1,456 @@
+// C-ABI runtime: contexts, HBM-resident volumes, frames and plain rendering.
+//
+// Replaces the reference's Vulkan plumbing on the render path:
+//   Renderer::render + RenderStatsCollector     -> xd_render (one launch, HIP events)
+//   Dda/SvoRaytraceResources (staging uploads)  -> xd_volume_from_grid / _svo
+//   HeadlessOutput::{synchronize, download}     -> xd_render's wait, xd_frame_download
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "xenodon.h"
+#include "xd_internal.hpp"
+#include "xd_runtime.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+}  // namespace
+
+namespace xd {
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+int fail_hip(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? XD_ERR_OUT_OF_MEMORY : XD_ERR_DEVICE, "%s: %s", what,
+                hipGetErrorString(e));
+}
+
+void clear_error() { g_error.clear(); }
+
+const char* traversal_name(int t) {
+    static const char* names[] = {"dda", "svo-naive", "esvo", "svo-df", "svo-rope"};
+    return (t >= 0 && t < 5) ? names[t] : "unknown";
+}
+
+int check_traversal(int traversal, const xd_volume* vol) {
+    if (traversal < XD_DDA || traversal > XD_SVO_ROPE) return fail(XD_ERR_INVALID_ARGUMENT, "Invalid shader '%d'", traversal);
+    const int need = traversal == XD_DDA ? XD_VOLUME_GRID : XD_VOLUME_SVO;
+    if (vol->kind != need) {
+        // wording of src/main_loop.cpp:120-125
+        return fail(XD_ERR_INCOMPATIBLE, "Shader '%s' is incompatible with model type '%s' (requires '%s')",
+                    traversal_name(traversal), vol->kind == XD_VOLUME_GRID ? "tiff" : "svo",
+                    need == XD_VOLUME_GRID ? "tiff" : "svo");
+    }
+    return XD_OK;
+}
+
+uint64_t iteration_cap(int traversal, const xd_volume* vol, const uint32_t model_dim[3]) {
+    // Same cap as the CPU restatement: it bounds every loop on well-formed
+    // input with a wide margin and only stops runaway rays.
+    if (traversal == XD_DDA) {
+        uint64_t s = 0;
+        for (int i = 0; i < 3; ++i) s += std::max<uint64_t>(vol->dims[i], model_dim[i]);
+        return 2 * s + 16;
+    }
+    uint32_t depth = std::min<uint32_t>(vol->max_depth, 24);
+    uint64_t side = std::max<uint64_t>(vol->side, 1ull << depth);
+    return 64 * side + (1ull << 18);
+}
+
+void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_camera* cam, const xd_uniforms* u) {
+    memset(&p, 0, sizeof(p));
+    p.cam.forward = f3{cam->forward[0], cam->forward[1], cam->forward[2]};
+    p.cam.up = f3{cam->up[0], cam->up[1], cam->up[2]};
+    p.cam.translation = f3{cam->translation[0], cam->translation[1], cam->translation[2]};
+    for (int i = 0; i < 2; ++i) {
+        p.view.out_off[i] = u->output_region.offset[i];
+        p.view.out_ext[i] = u->output_region.extent[i];
+        p.view.disp_off[i] = u->display_region.offset[i];
+        p.view.disp_ext[i] = u->display_region.extent[i];
+    }
+    p.view.ratio = f3{u->voxel_ratio[0], u->voxel_ratio[1], u->voxel_ratio[2]};
+    for (int i = 0; i < 3; ++i) p.view.model_dim[i] = u->model_dim[i];
+    p.view.emission = u->emission_coeff;
+    p.cap = iteration_cap(traversal, vol, u->model_dim);
+    if (vol->kind == XD_VOLUME_GRID) {
+        p.bricks = vol->d_bricks;
+        p.occ = vol->d_occ;
+        for (int i = 0; i < 3; ++i) {
+            p.dims[i] = (uint32_t)vol->dims[i];
+            p.nb[i] = vol->nb[i];
+        }
+    } else {
+        p.nodes = vol->d_nodes;
+        p.stack_slots = vol->max_depth + 1;
+    }
+}
+
+int check_view(const xd_uniforms* u) {
+    if (u->output_region.extent[0] == 0 || u->output_region.extent[1] == 0)
+        return fail(XD_ERR_INVALID_ARGUMENT, "output region is empty");
+    if (u->display_region.extent[0] == 0 || u->display_region.extent[1] == 0)
+        return fail(XD_ERR_INVALID_ARGUMENT, "display region is empty");
+    return XD_OK;
+}
+
+}  // namespace xd
+
+using namespace xd;
+
+extern "C" {
+
+const char* xd_last_error(void) { return g_error.c_str(); }
+
+int xd_device_count(int* count) {
+    if (!count) return fail(XD_ERR_INVALID_ARGUMENT, "count is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return fail_hip(e, "hipGetDeviceCount");
+    }
+    *count = n;
+    return XD_OK;
+}
+
+int xd_device_describe(int device_index, char* name, size_t name_len, uint32_t* pci_bus, uint32_t* pci_device,
+                       uint64_t* total_bytes) {
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device_index);
+    if (e != hipSuccess) return fail_hip(e, "hipGetDeviceProperties");
+    if (name && name_len) {
+        snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (pci_bus) *pci_bus = (uint32_t)prop.pciBusID;
+    if (pci_device) *pci_device = (uint32_t)prop.pciDeviceID;
+    if (total_bytes) *total_bytes = (uint64_t)prop.totalGlobalMem;
+    return XD_OK;
+}
+
+int xd_open(int device_index, xd_context** out) {
+    if (!out) return fail(XD_ERR_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(XD_ERR_DEVICE, "no HIP device available");
+    if (device_index < 0 || device_index >= n)
+        return fail(XD_ERR_INVALID_ARGUMENT, "device index %d out of range (%d devices)", device_index, n);
+    if ((e = hipSetDevice(device_index)) != hipSuccess) return fail_hip(e, "hipSetDevice");
+    auto* ctx = new xd_context();
+    ctx->device = device_index;
+    ctx->tile_w = 8;
+    if ((e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) {
+        delete ctx;
+        return fail_hip(e, "hipStreamCreate");
+    }
+    if ((e = hipEventCreate(&ctx->ev0)) != hipSuccess || (e = hipEventCreate(&ctx->ev1)) != hipSuccess ||
+        (e = hipMalloc(&ctx->d_words, sizeof(unsigned long long))) != hipSuccess) {
+        xd_close(ctx);
+        return fail_hip(e, "context setup");
+    }
+    clear_error();
+    *out = ctx;
+    return XD_OK;
+}
+
+int xd_close(xd_context* ctx) {
+    if (!ctx) return XD_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    // volumes and frames still alive are released with their context
+    std::vector<xd_volume*> vols(ctx->volumes.begin(), ctx->volumes.end());
+    for (auto* v : vols) xd_volume_free(v);
+    std::vector<xd_frame*> frs(ctx->frames.begin(), ctx->frames.end());
+    for (auto* f : frs) xd_frame_free(f);
+    if (ctx->d_words) (void)hipFree(ctx->d_words);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return XD_OK;
+}
+
+int xd_volume_from_grid(xd_context* ctx, const void* rgba8, uint64_t dim_x, uint64_t dim_y, uint64_t dim_z,
+                        xd_volume** out) {
+    if (!ctx || !rgba8 || !out) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (dim_x == 0 || dim_y == 0 || dim_z == 0) return fail(XD_ERR_INVALID_ARGUMENT, "grid has a zero dimension");
+    if (dim_x > (1u << 30) || dim_y > (1u << 30) || dim_z > (1u << 30))
+        return fail(XD_ERR_INVALID_ARGUMENT, "grid dimension above 2^30");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    auto* v = new xd_volume();
+    v->ctx = ctx;
+    v->kind = XD_VOLUME_GRID;
+    v->dims[0] = dim_x;
+    v->dims[1] = dim_y;
+    v->dims[2] = dim_z;
+    v->side = std::max({dim_x, dim_y, dim_z});
+    for (int i = 0; i < 3; ++i) v->nb[i] = (uint32_t)((v->dims[i] + kBrick - 1) / kBrick);
+    const uint64_t nbricks = (uint64_t)v->nb[0] * v->nb[1] * v->nb[2];
+    const uint64_t brick_bytes = nbricks * 64 * sizeof(uint32_t);
+    const uint32_t dims32[3] = {(uint32_t)dim_x, (uint32_t)dim_y, (uint32_t)dim_z};
+    uint32_t* staging = nullptr;
+    const uint64_t layer_texels = dim_x * dim_y;
+    if ((e = hipMalloc(&v->d_bricks, brick_bytes)) != hipSuccess || (e = hipMalloc(&v->d_occ, nbricks)) != hipSuccess ||
+        (e = hipMalloc(&staging, layer_texels * kBrick * sizeof(uint32_t))) != hipSuccess) {
+        if (staging) (void)hipFree(staging);
+        xd_volume_free(v);
+        return fail_hip(e, "grid allocation");
+    }
+    v->bytes = brick_bytes + nbricks;
+    const uint8_t* src = static_cast<const uint8_t*>(rgba8);
+    for (uint32_t bz = 0; bz < v->nb[2]; ++bz) {
+        const uint64_t z0 = (uint64_t)bz * kBrick;
+        const uint32_t layers = (uint32_t)std::min<uint64_t>(kBrick, dim_z - z0);
+        e = hipMemcpyAsync(staging, src + z0 * layer_texels * 4, layers * layer_texels * 4, hipMemcpyHostToDevice,
+                           ctx->stream);
+        if (e == hipSuccess) e = launch_brick_layer(staging, layers, bz, v->d_bricks, v->d_occ, dims32, v->nb, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(staging);
+            xd_volume_free(v);
+            return fail_hip(e, "grid upload");
+        }
+    }
+    (void)hipFree(staging);
+    ctx->volumes.insert(v);
+    clear_error();
+    *out = v;
+    return XD_OK;
+}
+
+// Structural depth of the tree below the root, and format checks.
+static int analyse_octree(const uint32_t* w, uint64_t n, uint32_t* max_depth, uint32_t* is_rope) {
+    bool ordered = true;
+    bool rope = false;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t* nd = w + i * kNodeWords;
+        const bool leaf = (nd[9] & kLeafMask) != 0;
+        for (int k = 0; k < 8; ++k) {
+            if (nd[k] >= n)
+                return fail(XD_ERR_FORMAT, "node %llu child %d = %u is out of range (%llu nodes)",
+                            (unsigned long long)i, k, nd[k], (unsigned long long)n);
+            if (!leaf && nd[k] <= i) ordered = false;
+            if (leaf && k < 6 && nd[k] != 0) rope = true;
+        }
+    }
+    *is_rope = rope ? 1u : 0u;
+    std::vector<uint8_t> height(n, 0);
+    if (ordered) {
+        // children have larger indices than their parent (reverse post-order,
+        // OctreeConstruction.h:92-112): one backward pass
+        for (uint64_t i = n; i-- > 0;) {
+            const uint32_t* nd = w + i * kNodeWords;
+            if (nd[9] & kLeafMask) continue;
+            uint32_t h = 0;
+            for (int k = 0; k < 8; ++k) h = std::max<uint32_t>(h, height[nd[k]]);
+            height[i] = (uint8_t)std::min<uint32_t>(h + 1, 255);
+        }
+    } else {
+        // general DAG: iterative DFS with cycle detection
+        std::vector<uint8_t> state(n, 0);  // 0 new, 1 on stack, 2 done
+        std::vector<std::pair<uint64_t, int>> st;
+        st.push_back({0, 0});
+        state[0] = 1;
+        while (!st.empty()) {
+            auto& top = st.back();
+            const uint32_t* nd = w + top.first * kNodeWords;
+            if ((nd[9] & kLeafMask) || top.second == 8) {
+                if (!(nd[9] & kLeafMask)) {
+                    uint32_t h = 0;
+                    for (int k = 0; k < 8; ++k) h = std::max<uint32_t>(h, height[nd[k]]);
+                    height[top.first] = (uint8_t)std::min<uint32_t>(h + 1, 255);
+                }
+                state[top.first] = 2;
+                st.pop_back();
+                continue;
+            }
+            const uint32_t c = nd[top.second++];
+            if (state[c] == 1) return fail(XD_ERR_FORMAT, "octree has a cycle through node %u", c);
+            if (state[c] == 0) {
+                state[c] = 1;
+                st.push_back({c, 0});
+            }
+        }
+    }
+    *max_depth = height[0];
+    if (*max_depth > 23) return fail(XD_ERR_FORMAT, "octree depth %u exceeds 23 levels", *max_depth);
+    return XD_OK;
+}
+
+int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, uint64_t side, xd_volume** out) {
+    if (!ctx || !nodes || !out) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (node_count == 0) return fail(XD_ERR_FORMAT, "octree has no nodes");
+    if (node_count > 0xFFFFFFFFull) return fail(XD_ERR_FORMAT, "octree has more than 2^32 nodes");
+    const uint32_t* w = static_cast<const uint32_t*>(nodes);
+    uint32_t max_depth = 0, is_rope = 0;
+    int rc = analyse_octree(w, node_count, &max_depth, &is_rope);
+    if (rc) return rc;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    auto* v = new xd_volume();
+    v->ctx = ctx;
+    v->kind = XD_VOLUME_SVO;
+    v->side = side;
+    v->dims[0] = v->dims[1] = v->dims[2] = side;
+    v->node_count = node_count;
+    v->max_depth = max_depth;
+    v->is_rope = is_rope;
+    const uint64_t bytes = node_count * kNodeWords * sizeof(uint32_t);
+    if ((e = hipMalloc(&v->d_nodes, bytes)) != hipSuccess) {
+        xd_volume_free(v);
+        return fail_hip(e, "octree allocation");
+    }
+    v->bytes = bytes;
+    if ((e = hipMemcpy(v->d_nodes, nodes, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+        xd_volume_free(v);
+        return fail_hip(e, "octree upload");
+    }
+    ctx->volumes.insert(v);
+    clear_error();
+    *out = v;
+    return XD_OK;
+}
+
+int xd_volume_info_get(const xd_volume* v, xd_volume_info* out) {
+    if (!v || !out) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    memset(out, 0, sizeof(*out));
+    out->kind = v->kind;
+    out->max_depth = v->max_depth;
+    for (int i = 0; i < 3; ++i) out->dims[i] = v->dims[i];
+    out->side = v->side;
+    out->node_count = v->node_count;
+    out->device_bytes = v->bytes;
+    out->is_rope = v->is_rope;
+    return XD_OK;
+}
+
+int xd_volume_free(xd_volume* v) {
+    if (!v) return XD_OK;
+    (void)hipSetDevice(v->ctx->device);
+    (void)hipStreamSynchronize(v->ctx->stream);
+    if (v->d_bricks) (void)hipFree(v->d_bricks);
+    if (v->d_occ) (void)hipFree(v->d_occ);
+    if (v->d_nodes) (void)hipFree(v->d_nodes);
+    v->ctx->volumes.erase(v);
+    delete v;
+    return XD_OK;
+}
+
+int xd_frame_create(xd_context* ctx, uint32_t width, uint32_t height, xd_frame** out) {
+    if (!ctx || !out) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (width == 0 || height == 0) return fail(XD_ERR_INVALID_ARGUMENT, "frame has a zero dimension");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    auto* f = new xd_frame();
+    f->ctx = ctx;
+    f->w = width;
+    f->h = height;
+    if ((e = hipMalloc(&f->d_pixels, (uint64_t)width * height * 4)) != hipSuccess) {
+        delete f;
+        return fail_hip(e, "frame allocation");
+    }
+    ctx->frames.insert(f);
+    *out = f;
+    return XD_OK;
+}
+
+int xd_frame_free(xd_frame* f) {
+    if (!f) return XD_OK;
+    (void)hipSetDevice(f->ctx->device);
+    (void)hipStreamSynchronize(f->ctx->stream);
+    (void)hipFree(f->d_pixels);
+    f->ctx->frames.erase(f);
+    delete f;
+    return XD_OK;
+}
+
+int xd_frame_device_ptr(xd_frame* f, void** out) {
+    if (!f || !out) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    *out = f->d_pixels;
+    return XD_OK;
+}
+
+int xd_render(xd_context* ctx, const xd_volume* vol, int traversal, const xd_camera* cam, const xd_uniforms* u,
+              xd_frame* frame, double* dispatch_ms) {
+    if (!ctx || !vol || !cam || !u || !frame) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_traversal(traversal, vol);
+    if (rc) return rc;
+    if ((rc = check_view(u))) return rc;
+    if (frame->w != u->output_region.extent[0] || frame->h != u->output_region.extent[1])
+        return fail(XD_ERR_INVALID_ARGUMENT, "frame is %ux%u but the output region is %ux%u", frame->w, frame->h,
+                    u->output_region.extent[0], u->output_region.extent[1]);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    RenderParams p;
+    fill_params(p, vol, traversal, cam, u);
+    p.out = frame->d_pixels;
+    p.out_pitch = frame->w;
+    if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return fail_hip(e, "hipEventRecord");
+    if ((e = launch_traversal(traversal, ctx->tile_w, false, p, ctx->stream)) != hipSuccess)
+        return fail_hip(e, "traversal launch");
+    if ((e = hipEventRecord(ctx->ev1, ctx->stream)) != hipSuccess) return fail_hip(e, "hipEventRecord");
+    if ((e = hipEventSynchronize(ctx->ev1)) != hipSuccess) return fail_hip(e, "traversal");
+    if (dispatch_ms) {
+        float ms = 0.0f;
+        if ((e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1)) != hipSuccess) return fail_hip(e, "hipEventElapsedTime");
+        *dispatch_ms = ms;
+    }
+    return XD_OK;
+}
+
+int xd_frame_download(xd_frame* f, uint32_t* dst, uint64_t stride_px) {
+    if (!f || !dst) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    if (stride_px == 0) stride_px = f->w;
+    if (stride_px < f->w) return fail(XD_ERR_INVALID_ARGUMENT, "stride below frame width");
+    hipError_t e = hipSetDevice(f->ctx->device);
+    if (e == hipSuccess) e = hipStreamSynchronize(f->ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpy2D(dst, stride_px * 4, f->d_pixels, (size_t)f->w * 4, (size_t)f->w * 4, f->h,
+                        hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail_hip(e, "frame download");
+    return XD_OK;
+}
+
+int xd_count_words(xd_context* ctx, const xd_volume* vol, int traversal, const xd_camera* cam, const xd_uniforms* u,
+                   uint64_t* words) {
+    if (!ctx || !vol || !cam || !u || !words) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_traversal(traversal, vol);
+    if (rc) return rc;
+    if ((rc = check_view(u))) return rc;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    RenderParams p;
+    fill_params(p, vol, traversal, cam, u);
+    p.words = ctx->d_words;
+    unsigned long long h = 0;
+    if ((e = hipMemcpyAsync(ctx->d_words, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+        return fail_hip(e, "counter reset");
+    if ((e = launch_traversal(traversal, ctx->tile_w, true, p, ctx->stream)) != hipSuccess)
+        return fail_hip(e, "counting launch");
+    if ((e = hipMemcpyAsync(&h, ctx->d_words, sizeof(h), hipMemcpyDeviceToHost, ctx->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+        return fail_hip(e, "counter read");
+    *words = h;
+    return XD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// Runtime object layouts shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <unordered_set>
+
+#include "xenodon.h"
+#include "xd_internal.hpp"
+
+struct xd_context {
+    int device = 0;
+    int tile_w = 8;  // per-wavefront tile width: 8 (8x8), 16 (16x4), 32 (32x2)
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned long long* d_words = nullptr;
+    // live objects created on this context; xd_close frees what is left
+    std::unordered_set<struct xd_volume*> volumes;
+    std::unordered_set<struct xd_frame*> frames;
+};
+
+struct xd_volume {
+    xd_context* ctx = nullptr;
+    int kind = XD_VOLUME_GRID;
+    uint64_t dims[3] = {0, 0, 0};
+    uint64_t side = 0;
+    uint64_t node_count = 0;
+    uint32_t max_depth = 0;
+    uint32_t is_rope = 0;
+    uint64_t bytes = 0;
+    // grid: 4x4x4 bricks + occupancy byte per brick
+    uint32_t* d_bricks = nullptr;
+    uint8_t* d_occ = nullptr;
+    uint32_t nb[3] = {0, 0, 0};
+    // octree: node_count * 10 words, the .svo node layout
+    uint32_t* d_nodes = nullptr;
+};
+
+struct xd_frame {
+    xd_context* ctx = nullptr;
+    uint32_t w = 0, h = 0;
+    uint32_t* d_pixels = nullptr;
+};
+
+namespace xd {
+
+int fail(int code, const char* fmt, ...);
+int fail_hip(hipError_t e, const char* what);
+void clear_error();
+const char* traversal_name(int t);
+int check_traversal(int traversal, const xd_volume* vol);
+int check_view(const xd_uniforms* u);
+uint64_t iteration_cap(int traversal, const xd_volume* vol, const uint32_t model_dim[3]);
+void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_camera* cam, const xd_uniforms* u);
+
+}  // namespace xd
