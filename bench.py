@@ -1,0 +1,230 @@
+"""Benchmark: Mrays/s of the five traversals at 1920x1080 on the 512x361x512
+CT-like volume over the reference's 150-frame benchmark camera path.
+
+A step renders one camera frame with each of the five traversals
+(dda on the grid; svo-naive, esvo, svo-df, svo-rope on its lossless rope
+octree), one launch per traversal per frame -- the reference's plain form
+(src/render/Renderer.cpp:55-103).  Inputs are HBM-resident before timing.
+
+Prints one JSON line: value = total rays / wall time of the K timed steps;
+per-traversal Mrays/s use the launches' own HIP-event time (the reference's
+RenderStats semantics, src/render/RenderStats.cpp:46-83).  The roofline
+object covers the dominant kernel: algorithmic bytes (4 B per volume word
+the reference algorithm reads, counted by the instrumented kernel variant on
+the same frames, + 4 B per pixel) over its mean launch time.  cpu_baseline
+is the oracle (CPU restatement of the same shaders) on the host cores over a
+bounded band of rows of the same frames.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from xenodon_amd import abi, synth  # noqa: E402
+
+CAMERA_FILE = os.path.join(ROOT, "tests", "golden", "camera.txt")
+WIDTH, HEIGHT = 1920, 1080
+EMISSION = 1.6  # <= 1 % saturated pixels on this volume and path
+TRAV_ORDER = ["dda", "svo-naive", "esvo", "svo-df", "svo-rope"]
+HBM_PEAK_GBS = 8000.0
+PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_ct512_1080p.json")
+
+
+def log(msg):
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def read_cameras(path):
+    """ScriptCameraController format: 9 floats per line (forward, up,
+    position), read until EOF (src/camera/ScriptCameraController.cpp:4-41)."""
+    vals = open(path).read().split()
+    if len(vals) % 9:
+        raise ValueError("Syntax error in camera input file")
+    v = np.array(vals, np.float32).reshape(-1, 9)
+    return [(r[0:3], r[3:6], r[6:9]) for r in v]
+
+
+def cpu_baseline(grid, nodes, side, cams, frames, threads, budget_s=30.0):
+    """The oracle on the host cores: full 1920x1080 frames of the same path
+    (frames evenly spaced over it), all five traversals per frame, stopping
+    after the frame that crosses budget_s seconds."""
+    from oracle import xo  # the checker, timed here only as the CPU baseline
+
+    og = xo.OracleVolume(grid=grid)
+    osv = xo.OracleVolume(nodes=nodes, side=side)
+    rays = 0
+    secs = 0.0
+    per_rays = {n: 0 for n in TRAV_ORDER}
+    per_secs = {n: 0.0 for n in TRAV_ORDER}
+    picked = [int(round(i * len(cams) / frames)) % len(cams) for i in range(frames)]
+    done = []
+    for k in picked:
+        f, up, eye = cams[k]
+        for name in TRAV_ORDER:
+            t = abi.TRAVERSALS[name]
+            vol, md = (og, list(grid.shape[2::-1][:3])) if t == abi.DDA else (osv, [side] * 3)
+            u = xo.make_uniforms(WIDTH, HEIGHT, md, emission=EMISSION)
+            t0 = time.perf_counter()
+            xo.render(t, vol, xo.make_camera(f, up, eye), u, threads=threads)
+            dt = time.perf_counter() - t0
+            rays += WIDTH * HEIGHT
+            secs += dt
+            per_rays[name] += WIDTH * HEIGHT
+            per_secs[name] += dt
+        done.append(k)
+        if secs > budget_s:
+            break
+    return {
+        "value": round(rays / secs / 1e6, 3),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle FP32 restatement on {threads} threads: camera frames {done} of the path at "
+                  f"{WIDTH}x{HEIGHT}, all 5 traversals each ({rays} rays, {secs:.1f} s)",
+        "per_traversal_mrays": {n: round(per_rays[n] / per_secs[n] / 1e6, 3) for n in TRAV_ORDER if per_secs[n]},
+    }
+
+
+def copy_calibration(gib):
+    """A plain device copy of known size, for scaling the FETCH_SIZE /
+    WRITE_SIZE counters of a rocprofv3 --pmc run."""
+    import torch
+
+    n = int(gib * (1 << 30)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=150, help="camera frames timed (each frame: 5 traversals)")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--traversals", default=",".join(TRAV_ORDER))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-frames", type=int, default=8, help="camera frames in the CPU-baseline sample")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run a device copy of this size (HBM counter calibration)")
+    ap.add_argument("--tile", type=int, default=0, help="per-wave tile width override (8/16/32)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        return 2
+    travs = [t for t in args.traversals.split(",") if t]
+
+    t0 = time.time()
+    grid = synth.ct_body()
+    log(f"volume 512x361x512 generated in {time.time() - t0:.1f}s, fnv64 {synth.volume_hash(grid):016x}")
+    t0 = time.time()
+    nodes, st = abi.build_octree(grid, tree=abi.TREE_ROPE)
+    side = st["side"]
+    log(f"lossless rope SVO: {st['node_count']} nodes, depth {st['depth']}, built in {time.time() - t0:.1f}s")
+    cams = read_cameras(CAMERA_FILE)
+
+    ctx = abi.Context(0)
+    if args.tile:
+        ctx.set_tile(args.tile)
+    gvol = ctx.grid_volume(grid)
+    svol = ctx.svo_volume(nodes, side)
+    frame = ctx.frame(WIDTH, HEIGHT)
+    vols = {"dda": (gvol, [512, 361, 512])}
+    for n in TRAV_ORDER[1:]:
+        vols[n] = (svol, [side] * 3)
+    log(f"resident: grid {gvol.info().device_bytes / 1e6:.0f} MB, svo {svol.info().device_bytes / 1e6:.0f} MB")
+
+    def uni(md):
+        return abi.make_uniforms(WIDTH, HEIGHT, md, emission=EMISSION)
+
+    def step(k, ms_acc):
+        f, up, eye = cams[k % len(cams)]
+        cam = abi.make_camera(f, up, eye)
+        for name in travs:
+            vol, md = vols[name]
+            ms = ctx.render(vol, abi.TRAVERSALS[name], cam, uni(md), frame)
+            if ms_acc is not None:
+                ms_acc[name].append(ms)
+
+    for k in range(args.warmup):
+        step(k, None)
+    ms_acc = {n: [] for n in travs}
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        step(k, ms_acc)
+    wall = time.perf_counter() - t_start
+    log(f"timed {args.steps} steps in {wall:.2f}s")
+
+    rays_per_frame = WIDTH * HEIGHT
+    total_rays = rays_per_frame * args.steps * len(travs)
+    per_trav = {n: rays_per_frame * len(v) / (sum(v) * 1e3) for n, v in ms_acc.items()}
+    mean_ms = {n: float(np.mean(v)) for n, v in ms_acc.items()}
+
+    # algorithmic bytes of every traversal over the same frames (untimed)
+    algo_bytes = {}
+    for name in travs:
+        vol, md = vols[name]
+        words = 0
+        for k in range(args.steps):
+            f, up, eye = cams[k % len(cams)]
+            words += ctx.count_words(vol, abi.TRAVERSALS[name], abi.make_camera(f, up, eye), uni(md))
+        algo_bytes[name] = (4 * words + 4 * rays_per_frame * args.steps) / args.steps
+    dom = max(travs, key=lambda n: sum(ms_acc[n]))
+    achieved = algo_bytes[dom] / (mean_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(PROFILE_SUMMARY):
+        prof = json.load(open(PROFILE_SUMMARY))
+        traffic = prof.get("hbm_bytes_per_launch", {}).get(dom)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "algorithmic_bytes_per_launch": round(algo_bytes[dom]), "mean_launch_ms": round(mean_ms[dom], 4),
+                "per_traversal_algorithmic_GBs": {n: round(algo_bytes[n] / (mean_ms[n] * 1e-3) / 1e9, 1)
+                                                  for n in travs}}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", 0) or min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(grid, nodes, side, cams, args.cpu_frames, threads)
+
+    value = total_rays / wall / 1e6
+    out = {
+        "metric": "Mrays/s per traversal @1080p, batched/served vs plain; p50/p99 request latency",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded CT-like 512x361x512 RGBA8 grid + its lossless rope SVO; reference camera.txt)",
+        "config": {"workload": "512x361x512 RGBA8 grid, all five traversals at 1920x1080, camera.txt path",
+                   "form": "plain (one launch per traversal per frame)", "emission_coeff": EMISSION,
+                   "traversals": travs, "frames": args.steps},
+        "per_traversal_mrays": {n: round(v, 1) for n, v in per_trav.items()},
+        "per_traversal_mean_ms": {n: round(v, 4) for n, v in mean_ms.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -111,6 +111,9 @@ XD_API int xd_open(int device_index, xd_context** out);
 /* Waits for the context's work, then frees the context together with every
  * volume and frame still alive on it; their handles become invalid. */
 XD_API int xd_close(xd_context* ctx);
+/* Per-wavefront pixel tile of subsequent launches: tile_w = 8 (8x8 tile),
+ * 16 (16x4) or 32 (32x2).  Results do not depend on it; speed does. */
+XD_API int xd_context_set_tile(xd_context* ctx, uint32_t tile_w);
 
 /* ---- volumes (replaces Dda/SvoRaytraceResources, src/render/) ------------ */
 
@@ -155,6 +158,29 @@ XD_API int xd_frame_download(xd_frame* frame, uint32_t* dst, uint64_t stride_px)
  * instrumented variant of the kernel with every shortcut disabled. */
 XD_API int xd_count_words(xd_context* ctx, const xd_volume* vol, int traversal, const xd_camera* cam,
                           const xd_uniforms* uniforms, uint64_t* words);
+
+/* ---- octree construction (`xenodon convert`, src/convert.cpp:13-122) ----- */
+
+typedef enum { XD_HEURISTIC_CHAN_DIFF = 0, XD_HEURISTIC_STD_DEV = 1 } xd_heuristic;
+typedef enum { XD_TREE_SPARSE = 0, XD_TREE_DAG = 1, XD_TREE_ROPE = 2 } xd_tree_type; /* Octree::Type */
+
+/* ConstructionStats (OctreeConstruction.h:50-62) plus the result's shape. */
+typedef struct {
+    uint64_t total_leaves;
+    uint64_t unique_leaves;
+    uint64_t total_nodes;
+    uint64_t depth;
+    uint64_t side;
+    uint64_t node_count;
+} xd_build_stats;
+
+/* build_octree(grid, stats, heuristic, type) (OctreeConstruction.h:227-237)
+ * on an RGBA8 grid (x fastest).  The node buffer (node_count * 10 u32 words,
+ * the .svo node layout) is bit-identical to the reference converter's; it is
+ * malloc'd into *out_nodes and released with xd_free. */
+XD_API int xd_build_octree(const void* rgba8, uint64_t dim_x, uint64_t dim_y, uint64_t dim_z, int heuristic,
+                           int chan_diff, double std_dev, int tree_type, uint32_t** out_nodes, xd_build_stats* stats);
+XD_API void xd_free(void* p);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,14 @@ class VolumeInfo(ctypes.Structure):
     ]
 
 
+class BuildStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("total_leaves", "unique_leaves", "total_nodes", "depth", "side", "node_count")]
+
+
+HEURISTIC_CHAN_DIFF, HEURISTIC_STD_DEV = 0, 1
+TREE_SPARSE, TREE_DAG, TREE_ROPE = 0, 1, 2
+
 _lib = None
 
 
@@ -73,6 +81,7 @@ def _declare(L):
                                 P(ctypes.c_uint32), P(ctypes.c_uint64)], ctypes.c_int),
         "xd_open": ([ctypes.c_int, P(vp)], ctypes.c_int),
         "xd_close": ([vp], ctypes.c_int),
+        "xd_context_set_tile": ([vp, ctypes.c_uint32], ctypes.c_int),
         "xd_volume_from_grid": ([vp, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, P(vp)], ctypes.c_int),
         "xd_volume_from_svo": ([vp, vp, ctypes.c_uint64, ctypes.c_uint64, P(vp)], ctypes.c_int),
         "xd_volume_info_get": ([vp, P(VolumeInfo)], ctypes.c_int),
@@ -83,6 +92,9 @@ def _declare(L):
         "xd_render": ([vp, vp, ctypes.c_int, P(Camera), P(Uniforms), vp, P(ctypes.c_double)], ctypes.c_int),
         "xd_frame_download": ([vp, vp, ctypes.c_uint64], ctypes.c_int),
         "xd_count_words": ([vp, vp, ctypes.c_int, P(Camera), P(Uniforms), P(ctypes.c_uint64)], ctypes.c_int),
+        "xd_build_octree": ([vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                             ctypes.c_double, ctypes.c_int, P(vp), P(BuildStats)], ctypes.c_int),
+        "xd_free": ([vp], None),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -185,6 +197,9 @@ class Context:
         check(lib().xd_frame_create(self.h, width, height, ctypes.byref(h)))
         return self._adopt(Frame(self, h, width, height))
 
+    def set_tile(self, tile_w):
+        check(lib().xd_context_set_tile(self.h, tile_w))
+
     def _adopt(self, child):
         self._children.add(child)
         return child
@@ -262,3 +277,20 @@ class Frame:
             self.free()
         except Exception:
             pass
+
+
+def build_octree(grid, heuristic=HEURISTIC_CHAN_DIFF, chan_diff=0, std_dev=0.0, tree=TREE_SPARSE):
+    """`xenodon convert` on a [Z, Y, X, 4] uint8 grid.  Returns (nodes uint32
+    [N, 10], stats dict); the node buffer is the reference converter's."""
+    g = np.ascontiguousarray(grid, dtype=np.uint8)
+    out = ctypes.c_void_p()
+    st = BuildStats()
+    check(lib().xd_build_octree(g.ctypes.data, g.shape[2], g.shape[1], g.shape[0], heuristic, chan_diff,
+                                float(std_dev), tree, ctypes.byref(out), ctypes.byref(st)))
+    n = st.node_count
+    try:
+        buf = (ctypes.c_uint32 * (n * 10)).from_address(out.value)
+        nodes = np.frombuffer(buf, dtype=np.uint32).reshape(n, 10).copy()
+    finally:
+        lib().xd_free(out)
+    return nodes, {f: getattr(st, f) for f, _ in BuildStats._fields_}

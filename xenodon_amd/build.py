@@ -20,6 +20,7 @@ BIN_DIR = os.path.join(PKG, "bin")
 ARCH = os.environ.get("XD_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["xd_kernels.hip", "xd_runtime.hip"]
+HOST_SOURCES = ["xd_convert.cpp"]
 
 # IEEE-strict float semantics (bit parity with the CPU restatement), and no
 # packed FP32 instructions (v_pk_*_f32 are kept out of generated code).
@@ -51,8 +52,13 @@ def build_library():
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         run([hipcc(), *HIPCC_FLAGS, "-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(objdir, src.replace(".cpp", ".o"))
+        run([hipcc(), "-O3", "-std=c++17", "-fPIC", "-pthread", "-fvisibility=hidden", "-Wall", "-ffp-contract=off",
+             "-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     tmp = LIB + ".tmp"
-    run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -168,6 +168,14 @@ int xd_open(int device_index, xd_context** out) {
     return XD_OK;
 }
 
+int xd_context_set_tile(xd_context* ctx, uint32_t tile_w) {
+    if (!ctx) return fail(XD_ERR_INVALID_ARGUMENT, "null context");
+    if (tile_w != 8 && tile_w != 16 && tile_w != 32)
+        return fail(XD_ERR_INVALID_ARGUMENT, "tile width %u is not 8, 16 or 32", tile_w);
+    ctx->tile_w = (int)tile_w;
+    return XD_OK;
+}
+
 int xd_close(xd_context* ctx) {
     if (!ctx) return XD_OK;
     (void)hipSetDevice(ctx->device);
