@@ -111,6 +111,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=150, help="camera frames timed (each frame: 5 traversals)")
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frame-stride", type=int, default=1, help="step k renders camera frame k*stride")
+    ap.add_argument("--batch-frames", type=int, default=10, help="camera frames per batched launch")
+    ap.add_argument("--per-traversal-groups", type=int, default=3,
+                    help="launches timed per traversal for the per-traversal batched rates")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--traversals", default=",".join(TRAV_ORDER))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8, help="camera frames in the CPU-baseline sample")
@@ -150,8 +155,11 @@ def main():
     def uni(md):
         return abi.make_uniforms(WIDTH, HEIGHT, md, emission=EMISSION)
 
+    def frame_of(k):
+        return cams[(k * args.frame_stride) % len(cams)]
+
     def step(k, ms_acc):
-        f, up, eye = cams[k % len(cams)]
+        f, up, eye = frame_of(k)
         cam = abi.make_camera(f, up, eye)
         for name in travs:
             vol, md = vols[name]
@@ -159,65 +167,134 @@ def main():
             if ms_acc is not None:
                 ms_acc[name].append(ms)
 
+    # ---------------- plain form: one launch per traversal per frame --------
     for k in range(args.warmup):
         step(k, None)
     ms_acc = {n: [] for n in travs}
     t_start = time.perf_counter()
     for k in range(args.steps):
         step(k, ms_acc)
-    wall = time.perf_counter() - t_start
-    log(f"timed {args.steps} steps in {wall:.2f}s")
-
+    plain_wall = time.perf_counter() - t_start
+    log(f"plain: {args.steps} steps in {plain_wall:.2f}s")
     rays_per_frame = WIDTH * HEIGHT
     total_rays = rays_per_frame * args.steps * len(travs)
+    plain_value = total_rays / plain_wall / 1e6
     per_trav = {n: rays_per_frame * len(v) / (sum(v) * 1e3) for n, v in ms_acc.items()}
     mean_ms = {n: float(np.mean(v)) for n, v in ms_acc.items()}
 
-    # algorithmic bytes of every traversal over the same frames (untimed)
-    algo_bytes = {}
+    # algorithmic bytes of every traversal for every frame (untimed)
+    words_of = {}
     for name in travs:
         vol, md = vols[name]
-        words = 0
         for k in range(args.steps):
-            f, up, eye = cams[k % len(cams)]
-            words += ctx.count_words(vol, abi.TRAVERSALS[name], abi.make_camera(f, up, eye), uni(md))
-        algo_bytes[name] = (4 * words + 4 * rays_per_frame * args.steps) / args.steps
-    dom = max(travs, key=lambda n: sum(ms_acc[n]))
-    achieved = algo_bytes[dom] / (mean_ms[dom] * 1e-3) / 1e9
+            f, up, eye = frame_of(k)
+            words_of[(name, k)] = ctx.count_words(vol, abi.TRAVERSALS[name], abi.make_camera(f, up, eye), uni(md))
+    algo_bytes = {n: sum(4 * words_of[(n, k)] + 4 * rays_per_frame for k in range(args.steps)) / args.steps
+                  for n in travs}
+    plain_roof = {n: round(algo_bytes[n] / (mean_ms[n] * 1e-3) / 1e9, 1) for n in travs}
+
+    # ---------------- batched form: one persistent launch per group --------
+    # a group is `batch_frames` consecutive steps x the traversals = views
+    B = max(1, min(args.batch_frames, args.steps))
+    frames_pool = [ctx.frame(WIDTH, HEIGHT) for _ in range(B * len(travs))]
+    batch = ctx.batch()
+
+    def group_views(k0):
+        views = []
+        for j in range(B):
+            f, up, eye = frame_of(k0 + j)
+            cam = abi.make_camera(f, up, eye)
+            for i, name in enumerate(travs):
+                vol, md = vols[name]
+                views.append((vol, abi.TRAVERSALS[name], cam, uni(md), frames_pool[j * len(travs) + i]))
+        return views
+
+    batch.set_views(group_views(0))
+    tuning = batch.autotune(trials=2) if not args.no_autotune else batch.tuning()
+    log(f"batched: tuning {tuning}")
+    for w in range(max(1, args.warmup // B)):
+        batch.set_views(group_views(w * B))
+        batch.render()
+    groups = [k0 for k0 in range(0, args.steps - B + 1, B)]
+    batch_ms = []
+    batch_rays = 0
+    t_start = time.perf_counter()
+    for k0 in groups:
+        batch.set_views(group_views(k0))
+        ms, rays = batch.render()
+        batch_ms.append(ms)
+        batch_rays += rays
+    batch_wall = time.perf_counter() - t_start
+    value = batch_rays / batch_wall / 1e6
+    log(f"batched: {len(groups)} launches of {B} steps in {batch_wall:.2f}s")
+    batch_algo = [sum(4 * words_of[(n, k0 + j)] + 4 * rays_per_frame for j in range(B) for n in travs)
+                  for k0 in groups]
+    mean_batch_ms = float(np.mean(batch_ms))
+    achieved = float(np.mean(batch_algo)) / (mean_batch_ms * 1e-3) / 1e9
+
+    # per-traversal batched rate: groups of one traversal only
+    per_trav_batched = {}
+    for name in travs:
+        vol, md = vols[name]
+        sub = ctx.batch()
+        sub.set_tuning(tuning["tile_w"], tuning["tiles_per_grab"], tuning["waves_per_simd"])
+        ms_sum = 0.0
+        rays_sum = 0
+        n_groups = max(1, min(len(groups), args.per_traversal_groups))
+        for g in range(n_groups):
+            k0 = groups[(g * len(groups)) // n_groups]  # spread over the path
+            views = []
+            for j in range(B * len(travs)):
+                f, up, eye = frame_of(k0 + j % B)
+                views.append((vol, abi.TRAVERSALS[name], abi.make_camera(f, up, eye), uni(md),
+                              frames_pool[j]))
+            views = views[:B]
+            sub.set_views(views)
+            sub.render()  # warm
+            ms, rays = sub.render()
+            ms_sum += ms
+            rays_sum += rays
+        per_trav_batched[name] = rays_sum / (ms_sum * 1e3)
+        sub.free()
+
     traffic = None
     if os.path.exists(PROFILE_SUMMARY):
         prof = json.load(open(PROFILE_SUMMARY))
-        traffic = prof.get("hbm_bytes_per_launch", {}).get(dom)
+        traffic = prof.get("hbm_bytes_per_launch", {}).get("batch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                "algorithmic_bytes_per_launch": round(algo_bytes[dom]), "mean_launch_ms": round(mean_ms[dom], 4),
-                "per_traversal_algorithmic_GBs": {n: round(algo_bytes[n] / (mean_ms[n] * 1e-3) / 1e9, 1)
-                                                  for n in travs}}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "xd::k_batch (persistent, all five traversals)",
+                "algorithmic_bytes_per_launch": round(float(np.mean(batch_algo))),
+                "mean_launch_ms": round(mean_batch_ms, 4),
+                "plain_per_traversal_algorithmic_GBs": plain_roof}
 
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", 0) or min(16, os.cpu_count() or 1))
         cpu = cpu_baseline(grid, nodes, side, cams, args.cpu_frames, threads)
 
-    value = total_rays / wall / 1e6
     out = {
         "metric": "Mrays/s per traversal @1080p, batched/served vs plain; p50/p99 request latency",
         "value": round(value, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": len(groups) * B,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "ms_per_step": round(batch_wall / (len(groups) * B) * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded CT-like 512x361x512 RGBA8 grid + its lossless rope SVO; reference camera.txt)",
         "config": {"workload": "512x361x512 RGBA8 grid, all five traversals at 1920x1080, camera.txt path",
-                   "form": "plain (one launch per traversal per frame)", "emission_coeff": EMISSION,
-                   "traversals": travs, "frames": args.steps},
-        "per_traversal_mrays": {n: round(v, 1) for n, v in per_trav.items()},
-        "per_traversal_mean_ms": {n: round(v, 4) for n, v in mean_ms.items()},
+                   "form": f"batched: persistent ray-tile queue, {B} frames x {len(travs)} traversals per "
+                           "hipGraph launch", "emission_coeff": EMISSION, "traversals": travs,
+                   "frames": len(groups) * B, "batch_tuning": tuning},
+        "batched_vs_plain": round(value / plain_value, 3),
+        "plain": {"value": round(plain_value, 2), "wall_s": round(plain_wall, 3),
+                  "per_traversal_mrays": {n: round(v, 1) for n, v in per_trav.items()},
+                  "per_traversal_mean_ms": {n: round(v, 4) for n, v in mean_ms.items()}},
+        "batched_per_traversal_mrays": {n: round(v, 1) for n, v in per_trav_batched.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -114,6 +114,12 @@ XD_API int xd_close(xd_context* ctx);
 /* Per-wavefront pixel tile of subsequent launches: tile_w = 8 (8x8 tile),
  * 16 (16x4) or 32 (32x2).  Results do not depend on it; speed does. */
 XD_API int xd_context_set_tile(xd_context* ctx, uint32_t tile_w);
+/* Named integer options of a context:
+ *   "octree_layout"  0 (default) derive the tagged device layout for trees of
+ *                    at most 2^30 nodes; 1 keep the .svo node words as they
+ *                    are (the layout used above 2^30 nodes).  Applies to
+ *                    volumes created afterwards; results are identical. */
+XD_API int xd_context_set_option(xd_context* ctx, const char* name, int64_t value);
 
 /* ---- volumes (replaces Dda/SvoRaytraceResources, src/render/) ------------ */
 
@@ -158,6 +164,45 @@ XD_API int xd_frame_download(xd_frame* frame, uint32_t* dst, uint64_t stride_px)
  * instrumented variant of the kernel with every shortcut disabled. */
 XD_API int xd_count_words(xd_context* ctx, const xd_volume* vol, int traversal, const xd_camera* cam,
                           const xd_uniforms* uniforms, uint64_t* words);
+
+/* ---- batched form: one persistent launch over many views ---------------- */
+
+/* One output of a batch: a volume, a traversal, the camera push constant and
+ * the uniforms of that output (output_region.extent = the frame's size). */
+typedef struct {
+    const xd_volume* volume;
+    int32_t traversal;
+    xd_camera camera;
+    xd_uniforms uniforms;
+    xd_frame* frame;
+} xd_view;
+
+/* Launch shape of a batch: per traversal the wavefront tile (8 = 8x8,
+ * 16 = 16x4, 32 = 32x2 pixels), the tiles a wavefront takes per queue grab
+ * (1..8), and the register budget of the persistent kernel (0 = compiler's
+ * choice, 8 = capped for 8 waves per SIMD).  Results never depend on it. */
+typedef struct {
+    uint32_t tile_w[5];
+    uint32_t tiles_per_grab;
+    uint32_t waves_per_simd;
+} xd_batch_tuning;
+
+typedef struct xd_batch xd_batch;
+
+XD_API int xd_batch_create(xd_context* ctx, xd_batch** out);
+/* Sets the batch's views (at most 4096; volumes and frames of the same
+ * context).  Uploads the view and tile tables; the captured hipGraph is reused
+ * unless the launch shape changes. */
+XD_API int xd_batch_set_views(xd_batch* batch, const xd_view* views, uint32_t count);
+XD_API int xd_batch_set_tuning(xd_batch* batch, const xd_batch_tuning* tuning);
+XD_API int xd_batch_get_tuning(const xd_batch* batch, xd_batch_tuning* tuning);
+/* Times every tile shape per traversal and every grab size on the current
+ * views (trials runs each, best kept) and keeps the fastest. */
+XD_API int xd_batch_autotune(xd_batch* batch, uint32_t trials, xd_batch_tuning* chosen);
+/* Replays the batch's graph and waits.  *ms is the HIP-event time of the
+ * whole launch; *rays the number of pixels rendered. */
+XD_API int xd_batch_render(xd_batch* batch, double* ms, uint64_t* rays);
+XD_API int xd_batch_free(xd_batch* batch);
 
 /* ---- octree construction (`xenodon convert`, src/convert.cpp:13-122) ----- */
 

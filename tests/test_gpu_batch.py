@@ -1,0 +1,120 @@
+"""Batched form (one persistent ray-tile-queue launch over many views): every
+frame is bit-identical to the plain one-launch render of the same view, for
+every launch shape the autotuner can pick."""
+import numpy as np
+import pytest
+
+from oracle import xo
+from tests import helpers as H
+from xenodon_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    if abi.device_count() < 1:
+        pytest.fail("no HIP device visible: GPU tests need an MI355X")
+    ctx = abi.Context(0)
+    shells = synth.shells_128()
+    nodes, st = abi.build_octree(shells, tree=abi.TREE_ROPE)
+    blocky = H.blocky_grid(64, 4, seed=41)
+    bnodes, bst = abi.build_octree(blocky, tree=abi.TREE_ROPE)
+    vols = {
+        "shells_grid": (ctx.grid_volume(shells), [128, 128, 128]),
+        "shells_svo": (ctx.svo_volume(nodes, st["side"]), [st["side"]] * 3),
+        "blocky_grid": (ctx.grid_volume(blocky), [64, 64, 64]),
+        "blocky_svo": (ctx.svo_volume(bnodes, bst["side"]), [bst["side"]] * 3),
+    }
+    yield ctx, vols
+    ctx.close()
+
+
+def _views(ctx, vols, seed):
+    rng = np.random.default_rng(seed)
+    plan = [("shells_grid", abi.DDA), ("shells_svo", abi.ESVO), ("shells_svo", abi.SVO_DF),
+            ("shells_svo", abi.SVO_NAIVE), ("shells_svo", abi.SVO_ROPE), ("blocky_grid", abi.DDA),
+            ("blocky_svo", abi.SVO_DF), ("blocky_svo", abi.ESVO), ("blocky_svo", abi.SVO_ROPE),
+            ("shells_svo", abi.SVO_DF)]
+    sizes = [(256, 256), (300, 200), (64, 80), (1, 1), (320, 240), (77, 131), (200, 150), (128, 96), (33, 65),
+             (160, 120)]
+    out = []
+    cams = H.orbit_cameras(len(plan), 1.8, seed=seed)
+    for (vname, t), (W, Hh), (f, up, eye) in zip(plan, sizes, cams):
+        vol, md = vols[vname]
+        e = float(rng.uniform(1.0, 4.0))
+        out.append((vol, t, abi.make_camera(f, up, eye), abi.make_uniforms(W, Hh, md, emission=e), ctx.frame(W, Hh)))
+    return out
+
+
+def _check_against_plain(ctx, views):
+    for vol, t, cam, uni, fr in views:
+        got = fr.download()
+        ref, _ = ctx.render_image(vol, t, cam, uni)
+        assert np.array_equal(got, ref), abi.TRAVERSAL_NAMES[t]
+
+
+def test_batch_matches_plain_renders(env):
+    ctx, vols = env
+    b = ctx.batch()
+    views = _views(ctx, vols, seed=1)
+    b.set_views(views)
+    ms, rays = b.render()
+    assert rays == sum(v[3].output_region.extent[0] * v[3].output_region.extent[1] for v in views)
+    assert ms > 0
+    _check_against_plain(ctx, views)
+
+
+@pytest.mark.parametrize("tile_w,grab,waves", [(16, 1, 0), (32, 2, 8), (8, 4, 8)])
+def test_batch_launch_shapes_do_not_change_pixels(env, tile_w, grab, waves):
+    ctx, vols = env
+    b = ctx.batch()
+    b.set_tuning([tile_w, 8, tile_w, 32, 16], grab, waves)
+    views = _views(ctx, vols, seed=2)
+    b.set_views(views)
+    b.render()
+    _check_against_plain(ctx, views)
+
+
+def test_batch_reuses_graph_with_new_cameras_and_autotunes(env):
+    ctx, vols = env
+    b = ctx.batch()
+    views = _views(ctx, vols, seed=3)
+    b.set_views(views)
+    b.render()
+    chosen = b.autotune(trials=1)
+    assert all(w in (8, 16, 32) for w in chosen["tile_w"]) and 1 <= chosen["tiles_per_grab"] <= 8
+    views2 = _views(ctx, vols, seed=4)  # new cameras, sizes kept
+    b.set_views(views2)
+    for _ in range(3):  # graph replays, each checked against plain renders
+        b.render()
+        _check_against_plain(ctx, views2)
+
+
+def test_batch_matches_oracle_on_c1(env):
+    ctx, vols = env
+    grid = synth.shells_128()
+    ov = xo.OracleVolume(grid=grid)
+    vol, md = vols["shells_grid"]
+    b = ctx.batch()
+    cams = H.orbit_cameras(4, 2 * np.sqrt(3), seed=7)
+    views = [(vol, abi.DDA, abi.make_camera(*c), abi.make_uniforms(256, 256, md, emission=4.0), ctx.frame(256, 256))
+             for c in cams]
+    b.set_views(views)
+    b.render()
+    for (v, t, cam, uni, fr), c in zip(views, cams):
+        ref = xo.render(xo.DDA, ov, xo.make_camera(*c), xo.make_uniforms(256, 256, md, emission=4.0))["rgba"]
+        assert np.array_equal(fr.download(), ref)
+
+
+def test_batch_rejects_bad_views(env):
+    ctx, vols = env
+    b = ctx.batch()
+    vol, md = vols["shells_svo"]
+    with pytest.raises(abi.XdError) as e:
+        b.set_views([(vol, abi.DDA, abi.make_camera((0, 0, 1), (0, 1, 0), (0.5, 0.5, -2)),
+                      abi.make_uniforms(8, 8, md), ctx.frame(8, 8))])
+    assert e.value.status == 4
+    with pytest.raises(abi.XdError):
+        b.set_views([(vol, abi.ESVO, abi.make_camera((0, 0, 1), (0, 1, 0), (0.5, 0.5, -2)),
+                      abi.make_uniforms(8, 8, md), ctx.frame(9, 8))])

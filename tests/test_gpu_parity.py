@@ -192,3 +192,45 @@ def test_malformed_octree_is_rejected(ctx):
     with pytest.raises(abi.XdError) as e:
         ctx.svo_volume(nodes, 2)
     assert "cycle" in str(e.value)
+
+
+@pytest.mark.parametrize("t", [abi.SVO_NAIVE, abi.ESVO, abi.SVO_DF, abi.SVO_ROPE])
+def test_raw_octree_layout_bit_exact(t):
+    """Trees above 2^30 nodes keep the .svo node words on the device; the
+    context option forces that layout so its kernels are checked here."""
+    with abi.Context(0) as ctx:
+        ctx.set_option("octree_layout", 1)
+        grid = synth.shells_128()
+        nodes, st = xo.build_octree(grid, tree=xo.TREE_ROPE)
+        gv = ctx.svo_volume(nodes, st["side"])
+        ov = xo.OracleVolume(nodes=nodes, side=st["side"])
+        for k, (f, up, eye) in enumerate(c1_orbit()[:4]):
+            u_args = (200, 150, [st["side"]] * 3)
+            img, _ = ctx.render_image(gv, t, abi.make_camera(f, up, eye), abi.make_uniforms(*u_args, emission=4.0))
+            ref = xo.render(t, ov, xo.make_camera(f, up, eye), xo.make_uniforms(*u_args, emission=4.0))
+            _assert_parity(img, ref["rgba"], f"raw/{abi.TRAVERSAL_NAMES[t]}/frame{k}")
+            words = ctx.count_words(gv, t, abi.make_camera(f, up, eye), abi.make_uniforms(*u_args, emission=4.0))
+            ref_w = xo.render(t, ov, xo.make_camera(f, up, eye), xo.make_uniforms(*u_args, emission=4.0),
+                              words=True)["words"].sum()
+            assert words == int(ref_w)
+
+
+@pytest.mark.parametrize("t", [abi.SVO_NAIVE, abi.ESVO, abi.SVO_DF, abi.SVO_ROPE])
+def test_dag_and_single_leaf_trees_bit_exact(ctx, t):
+    """--dag trees share subtrees between parents (the device renumbering
+    keeps a shared node's first index); a uniform volume is one leaf root."""
+    grids = {"dag": H.blocky_grid(64, 4, seed=31), "uniform": np.full((16, 16, 16, 4), 90, np.uint8)}
+    for name, g in grids.items():
+        nodes, st = xo.build_octree(g, tree=xo.TREE_DAG if name == "dag" else xo.TREE_ROPE)
+        if name == "dag":
+            sparse, _ = xo.build_octree(g)
+            assert nodes.shape[0] < sparse.shape[0]
+        else:
+            assert nodes.shape[0] == 1
+        gv = ctx.svo_volume(nodes, st["side"])
+        ov = xo.OracleVolume(nodes=nodes, side=st["side"])
+        for k, (f, up, eye) in enumerate(H.orbit_cameras(2, 1.7, seed=9) + [H.look_at((0.4, 0.6, 0.5), (1, 0, 1))]):
+            args = (72, 56, [st["side"]] * 3)
+            img, _ = ctx.render_image(gv, t, abi.make_camera(f, up, eye), abi.make_uniforms(*args, emission=1.5))
+            ref = xo.render(t, ov, xo.make_camera(f, up, eye), xo.make_uniforms(*args, emission=1.5))["rgba"]
+            _assert_parity(img, ref, f"{name}/{abi.TRAVERSAL_NAMES[t]}/cam{k}")

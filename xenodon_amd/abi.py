@@ -60,6 +60,16 @@ class VolumeInfo(ctypes.Structure):
     ]
 
 
+class View(ctypes.Structure):
+    _fields_ = [("volume", ctypes.c_void_p), ("traversal", ctypes.c_int32), ("camera", Camera),
+                ("uniforms", Uniforms), ("frame", ctypes.c_void_p)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("tile_w", ctypes.c_uint32 * 5), ("tiles_per_grab", ctypes.c_uint32),
+                ("waves_per_simd", ctypes.c_uint32)]
+
+
 class BuildStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
                 ("total_leaves", "unique_leaves", "total_nodes", "depth", "side", "node_count")]
@@ -82,6 +92,7 @@ def _declare(L):
         "xd_open": ([ctypes.c_int, P(vp)], ctypes.c_int),
         "xd_close": ([vp], ctypes.c_int),
         "xd_context_set_tile": ([vp, ctypes.c_uint32], ctypes.c_int),
+        "xd_context_set_option": ([vp, ctypes.c_char_p, ctypes.c_int64], ctypes.c_int),
         "xd_volume_from_grid": ([vp, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, P(vp)], ctypes.c_int),
         "xd_volume_from_svo": ([vp, vp, ctypes.c_uint64, ctypes.c_uint64, P(vp)], ctypes.c_int),
         "xd_volume_info_get": ([vp, P(VolumeInfo)], ctypes.c_int),
@@ -95,6 +106,13 @@ def _declare(L):
         "xd_build_octree": ([vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
                              ctypes.c_double, ctypes.c_int, P(vp), P(BuildStats)], ctypes.c_int),
         "xd_free": ([vp], None),
+        "xd_batch_create": ([vp, P(vp)], ctypes.c_int),
+        "xd_batch_set_views": ([vp, P(View), ctypes.c_uint32], ctypes.c_int),
+        "xd_batch_set_tuning": ([vp, P(BatchTuning)], ctypes.c_int),
+        "xd_batch_get_tuning": ([vp, P(BatchTuning)], ctypes.c_int),
+        "xd_batch_autotune": ([vp, ctypes.c_uint32, P(BatchTuning)], ctypes.c_int),
+        "xd_batch_render": ([vp, P(ctypes.c_double), P(ctypes.c_uint64)], ctypes.c_int),
+        "xd_batch_free": ([vp], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -197,8 +215,16 @@ class Context:
         check(lib().xd_frame_create(self.h, width, height, ctypes.byref(h)))
         return self._adopt(Frame(self, h, width, height))
 
+    def batch(self):
+        h = ctypes.c_void_p()
+        check(lib().xd_batch_create(self.h, ctypes.byref(h)))
+        return self._adopt(Batch(self, h))
+
     def set_tile(self, tile_w):
         check(lib().xd_context_set_tile(self.h, tile_w))
+
+    def set_option(self, name, value):
+        check(lib().xd_context_set_option(self.h, name.encode(), int(value)))
 
     def _adopt(self, child):
         self._children.add(child)
@@ -226,6 +252,62 @@ class Context:
             return f.download(), ms
         finally:
             f.free()
+
+
+class Batch:
+    """Persistent ray-tile-queue launch over many views (xd_batch_*)."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+        self._keep = []
+
+    def set_views(self, views):
+        """views: list of (volume, traversal, camera, uniforms, frame)."""
+        arr = (View * max(1, len(views)))()
+        for i, (vol, t, cam, uni, fr) in enumerate(views):
+            arr[i].volume = vol.h
+            arr[i].traversal = t
+            arr[i].camera = cam
+            arr[i].uniforms = uni
+            arr[i].frame = fr.h
+        self._keep = list(views)
+        check(lib().xd_batch_set_views(self.h, arr, len(views)))
+
+    def tuning(self):
+        t = BatchTuning()
+        check(lib().xd_batch_get_tuning(self.h, ctypes.byref(t)))
+        return {"tile_w": list(t.tile_w), "tiles_per_grab": t.tiles_per_grab, "waves_per_simd": t.waves_per_simd}
+
+    def set_tuning(self, tile_w, tiles_per_grab=1, waves_per_simd=0):
+        t = BatchTuning()
+        t.tile_w[:] = list(tile_w)
+        t.tiles_per_grab = tiles_per_grab
+        t.waves_per_simd = waves_per_simd
+        check(lib().xd_batch_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self, trials=3):
+        t = BatchTuning()
+        check(lib().xd_batch_autotune(self.h, trials, ctypes.byref(t)))
+        return {"tile_w": list(t.tile_w), "tiles_per_grab": t.tiles_per_grab, "waves_per_simd": t.waves_per_simd}
+
+    def render(self):
+        """Returns (ms, rays)."""
+        ms = ctypes.c_double()
+        rays = ctypes.c_uint64()
+        check(lib().xd_batch_render(self.h, ctypes.byref(ms), ctypes.byref(rays)))
+        return ms.value, rays.value
+
+    def free(self):
+        if self.h:
+            lib().xd_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Volume:

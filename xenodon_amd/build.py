@@ -19,8 +19,8 @@ LIB = os.path.join(PKG, "libxenodon_amd.so")
 BIN_DIR = os.path.join(PKG, "bin")
 ARCH = os.environ.get("XD_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["xd_kernels.hip", "xd_runtime.hip"]
-HOST_SOURCES = ["xd_convert.cpp"]
+HIP_SOURCES = ["xd_kernels.hip", "xd_runtime.hip", "xd_batch.hip"]
+HOST_SOURCES = ["xd_convert.cpp", "xd_layout.cpp"]
 
 # IEEE-strict float semantics (bit parity with the CPU restatement), and no
 # packed FP32 instructions (v_pk_*_f32 are kept out of generated code).

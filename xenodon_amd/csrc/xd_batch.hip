@@ -1,0 +1,465 @@
+// Batched form: one persistent launch drains a global ray-tile queue that
+// spans many views (any mix of sizes, volumes and traversals).
+//
+// Queue: every view is cut into wavefront tiles (8x8, 16x4 or 32x2 pixels,
+// chosen per traversal); a tile is one u32 (view << 20 | tile).  Tiles are
+// dealt into 8 ranges by the band of the view they lie in, one range per XCD
+// (the hardware deals workgroups to the XCDs round-robin, so block b runs on
+// XCD b % 8).  A wavefront takes tiles from its own XCD's range with one
+// atomic per grab, and when that range is exhausted it moves on to the other
+// ranges, so the launch ends only when every tile is done and no CU idles
+// while work is left.  Each tile is rendered by exactly the per-ray code of
+// the plain kernels, so every pixel is identical to the plain form.
+//
+// The launch (counter reset + persistent kernel) is captured once into a
+// hipGraph and replayed for every batch; new cameras or views only rewrite
+// the device-side view and tile tables.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "xd_runtime.hpp"
+#include "xd_trace.hpp"
+
+namespace xd {
+
+struct BatchView {
+    RenderParams p;
+    uint32_t traversal;
+    uint32_t tw_shift;  // log2 of the tile width
+    uint32_t tiles_x;
+    uint32_t pad;
+};
+
+constexpr uint32_t kQueues = 8;
+constexpr uint32_t kMaxBatchViews = 1u << 12;
+constexpr uint32_t kMaxTilesPerView = 1u << 20;
+
+__global__ void k_batch_reset(uint32_t* counters) {
+    if (threadIdx.x < kQueues) counters[threadIdx.x] = 0;
+}
+
+__device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds) {
+    const uint32_t sh = V.tw_shift;
+    const uint32_t tx = tile % V.tiles_x, ty = tile / V.tiles_x;
+    const uint32_t ix = (tx << sh) + (lane & ((1u << sh) - 1u));
+    const uint32_t iy = ty * (64u >> sh) + (lane >> sh);
+    const RenderParams& p = V.p;
+    if (ix >= p.view.out_ext[0] || iy >= p.view.out_ext[1]) return;
+    const f3 rd = ray_dir(p.cam, p.view, ix, iy);
+    uint64_t words = 0;
+    f3 c;
+    float coeff;
+    switch (V.traversal) {
+    case 0: {
+        const float side = max_elem(mk((float)p.dims[0], (float)p.dims[1], (float)p.dims[2]));
+        const f3 ro = muls(p.cam.translation, side);
+        coeff = emission_coeff(p.view, rd) / side;
+        c = p.wide ? dda_trace<false, true>(p, p.macro, ro, rd, words)
+                   : dda_trace<false, false>(p, p.macro, ro, rd, words);
+        break;
+    }
+    case 1:
+        c = p.cw ? svo_pixel<1, false, true>(p, rd, lds, nullptr, words)
+                 : svo_pixel<1, false, false>(p, rd, lds, nullptr, words);
+        coeff = emission_coeff(p.view, rd);
+        break;
+    case 2:
+        c = p.cw ? svo_pixel<2, false, true>(p, rd, lds, nullptr, words)
+                 : svo_pixel<2, false, false>(p, rd, lds, nullptr, words);
+        coeff = emission_coeff(p.view, rd);
+        break;
+    case 3:
+        c = p.cw ? svo_pixel<3, false, true>(p, rd, lds, nullptr, words)
+                 : svo_pixel<3, false, false>(p, rd, lds, nullptr, words);
+        coeff = emission_coeff(p.view, rd);
+        break;
+    default:
+        c = p.cw ? svo_pixel<4, false, true>(p, rd, lds, nullptr, words)
+                 : svo_pixel<4, false, false>(p, rd, lds, nullptr, words);
+        coeff = emission_coeff(p.view, rd);
+        break;
+    }
+    p.out[(uint64_t)iy * p.out_pitch + ix] = pack_rgba(muls(c, coeff));
+}
+
+__device__ __forceinline__ void batch_body(const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles,
+                                           const uint32_t* __restrict__ qbeg, uint32_t* __restrict__ counters,
+                                           uint32_t per_grab, uint32_t* lds) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t home = blockIdx.x % kQueues;
+    for (uint32_t qi = 0; qi < kQueues; ++qi) {
+        const uint32_t q = (home + qi) % kQueues;
+        const uint32_t beg = qbeg[q], end = qbeg[q + 1];
+        for (;;) {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(&counters[q], per_grab);
+            t = __shfl(t, 0, 64);
+            if (beg + t >= end) break;  // every wave leaves once each range is drained
+            const uint32_t stop = min(end, beg + t + per_grab);
+            for (uint32_t i = beg + t; i < stop; ++i) {
+                const uint32_t e = tiles[i];
+                shade_tile(views[e >> 20], e & (kMaxTilesPerView - 1u), lane, lds);
+            }
+        }
+    }
+}
+
+// Register budget left to the compiler (about 6 waves per SIMD) ...
+__global__ __launch_bounds__(256) void k_batch(const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles,
+                                               const uint32_t* __restrict__ qbeg, uint32_t* __restrict__ counters,
+                                               uint32_t per_grab) {
+    extern __shared__ uint32_t lds[];
+    batch_body(views, tiles, qbeg, counters, per_grab, lds);
+}
+
+// ... or capped for 8 waves per SIMD (a few registers spill).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_w8(
+    const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ qbeg,
+    uint32_t* __restrict__ counters, uint32_t per_grab) {
+    extern __shared__ uint32_t lds[];
+    batch_body(views, tiles, qbeg, counters, per_grab, lds);
+}
+
+}  // namespace xd
+
+using namespace xd;
+
+struct xd_batch {
+    xd_context* ctx = nullptr;
+    std::vector<xd_view> views;
+    xd_batch_tuning tuning{};
+    // device tables
+    BatchView* d_views = nullptr;
+    uint32_t* d_tiles = nullptr;
+    uint32_t* d_qbeg = nullptr;
+    uint32_t* d_counters = nullptr;
+    uint32_t view_cap = 0, tile_cap = 0;
+    uint64_t total_rays = 0;
+    // captured launch
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t graph_slots = 0, graph_grab = 0, graph_w8 = 0;  // configuration the graph was captured with
+    uint32_t grid = 0;
+};
+
+namespace {
+
+uint32_t tw_shift_of(uint32_t tw) { return tw == 32 ? 5u : (tw == 16 ? 4u : 3u); }
+
+void drop_graph(xd_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int ensure_capacity(xd_batch* b, uint32_t nviews, uint32_t ntiles) {
+    hipError_t e;
+    if (nviews > b->view_cap) {
+        drop_graph(b);
+        if (b->d_views) (void)hipFree(b->d_views);
+        b->d_views = nullptr;
+        const uint32_t cap = std::max<uint32_t>(nviews, 64);
+        if ((e = hipMalloc(&b->d_views, sizeof(BatchView) * cap)) != hipSuccess) return fail_hip(e, "batch views");
+        b->view_cap = cap;
+    }
+    if (ntiles > b->tile_cap) {
+        drop_graph(b);
+        if (b->d_tiles) (void)hipFree(b->d_tiles);
+        b->d_tiles = nullptr;
+        const uint32_t cap = std::max<uint32_t>(ntiles, 1u << 16);
+        if ((e = hipMalloc(&b->d_tiles, sizeof(uint32_t) * cap)) != hipSuccess) return fail_hip(e, "batch tiles");
+        b->tile_cap = cap;
+    }
+    return XD_OK;
+}
+
+// Builds and uploads the view and tile tables for the current views/tuning.
+int upload_tables(xd_batch* b, uint32_t* max_slots) {
+    const uint32_t n = (uint32_t)b->views.size();
+    std::vector<BatchView> bv(n);
+    std::vector<std::vector<uint32_t>> q(kQueues);
+    uint32_t slots = 1;
+    b->total_rays = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const xd_view& v = b->views[i];
+        BatchView& B = bv[i];
+        memset(&B, 0, sizeof(B));
+        fill_params(B.p, v.volume, v.traversal, &v.camera, &v.uniforms);
+        B.p.top_nodes = 0;  // no LDS top cache: views may use different volumes
+        B.p.out = v.frame->d_pixels;
+        B.p.out_pitch = v.frame->w;
+        B.traversal = (uint32_t)v.traversal;
+        uint32_t tw = b->tuning.tile_w[v.traversal];
+        if (tw != 8 && tw != 16 && tw != 32) tw = 8;
+        B.tw_shift = tw_shift_of(tw);
+        const uint32_t th = 64u / tw;
+        const uint32_t W = v.uniforms.output_region.extent[0], H = v.uniforms.output_region.extent[1];
+        B.tiles_x = (W + tw - 1) / tw;
+        const uint32_t tiles_y = (H + th - 1) / th;
+        const uint64_t nt = (uint64_t)B.tiles_x * tiles_y;
+        if (nt > kMaxTilesPerView)
+            return fail(XD_ERR_INVALID_ARGUMENT, "view %u has %llu tiles (limit %u)", i, (unsigned long long)nt,
+                        kMaxTilesPerView);
+        if (v.traversal == XD_ESVO || v.traversal == XD_SVO_DF) slots = std::max(slots, B.p.stack_slots);
+        for (uint32_t ty = 0; ty < tiles_y; ++ty) {
+            const uint32_t band = (uint32_t)((uint64_t)ty * kQueues / tiles_y);
+            for (uint32_t tx = 0; tx < B.tiles_x; ++tx) q[band].push_back((i << 20) | (ty * B.tiles_x + tx));
+        }
+        b->total_rays += (uint64_t)W * H;
+    }
+    // one LDS layout for the whole launch: every wave's stack region is sized
+    // for the deepest tree of the batch
+    for (uint32_t i = 0; i < n; ++i) bv[i].p.lds_slots = slots;
+    uint32_t qbeg[kQueues + 1];
+    uint32_t total = 0;
+    for (uint32_t k = 0; k < kQueues; ++k) {
+        qbeg[k] = total;
+        total += (uint32_t)q[k].size();
+    }
+    qbeg[kQueues] = total;
+    int rc = ensure_capacity(b, n, std::max<uint32_t>(total, 1));
+    if (rc) return rc;
+    std::vector<uint32_t> flat;
+    flat.reserve(total);
+    for (auto& v : q) flat.insert(flat.end(), v.begin(), v.end());
+    hipError_t e;
+    if ((e = hipMemcpy(b->d_views, bv.data(), sizeof(BatchView) * n, hipMemcpyHostToDevice)) != hipSuccess ||
+        (total && (e = hipMemcpy(b->d_tiles, flat.data(), sizeof(uint32_t) * total, hipMemcpyHostToDevice)) != hipSuccess) ||
+        (e = hipMemcpy(b->d_qbeg, qbeg, sizeof(qbeg), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail_hip(e, "batch tables");
+    *max_slots = slots;
+    return XD_OK;
+}
+
+int capture(xd_batch* b, uint32_t slots) {
+    drop_graph(b);
+    hipError_t e;
+    const size_t lds = (size_t)2 * slots * kBlockThreads * sizeof(uint32_t);  // 4 waves x own stacks
+    int dev = b->ctx->device, cus = 0, per_cu = 0;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess ||
+        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+             &per_cu, reinterpret_cast<const void*>(b->tuning.waves_per_simd == 8 ? k_batch_w8 : k_batch), 256,
+             lds)) != hipSuccess)
+        return fail_hip(e, "batch occupancy");
+    b->grid = (uint32_t)std::max(1, cus * std::max(per_cu, 1));
+    b->grid = (b->grid + kQueues - 1) / kQueues * kQueues;
+    hipStream_t s = b->ctx->stream;
+    const uint32_t grab = std::max<uint32_t>(1, std::min<uint32_t>(b->tuning.tiles_per_grab, 8));
+    if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return fail_hip(e, "hipStreamBeginCapture");
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->d_counters);
+    if (b->tuning.waves_per_simd == 8)
+        hipLaunchKernelGGL(k_batch_w8, dim3(b->grid), dim3(256), lds, s, b->d_views, b->d_tiles, b->d_qbeg,
+                           b->d_counters, grab);
+    else
+        hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(256), lds, s, b->d_views, b->d_tiles, b->d_qbeg,
+                           b->d_counters, grab);
+    hipError_t le = hipGetLastError();
+    if ((e = hipStreamEndCapture(s, &b->graph)) != hipSuccess) return fail_hip(e, "hipStreamEndCapture");
+    if (le != hipSuccess) return fail_hip(le, "batch launch capture");
+    if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
+        return fail_hip(e, "hipGraphInstantiate");
+    b->graph_slots = slots;
+    b->graph_grab = grab;
+    b->graph_w8 = b->tuning.waves_per_simd == 8;
+    return XD_OK;
+}
+
+int prepare(xd_batch* b) {
+    uint32_t slots = 1;
+    int rc = upload_tables(b, &slots);
+    if (rc) return rc;
+    const uint32_t grab = std::max<uint32_t>(1, std::min<uint32_t>(b->tuning.tiles_per_grab, 8));
+    if (!b->exec || slots > b->graph_slots || grab != b->graph_grab ||
+        b->graph_w8 != (uint32_t)(b->tuning.waves_per_simd == 8))
+        return capture(b, slots);
+    return XD_OK;
+}
+
+int run(xd_batch* b, double* ms) {
+    hipStream_t s = b->ctx->stream;
+    hipError_t e;
+    if ((e = hipEventRecord(b->ctx->ev0, s)) != hipSuccess || (e = hipGraphLaunch(b->exec, s)) != hipSuccess ||
+        (e = hipEventRecord(b->ctx->ev1, s)) != hipSuccess || (e = hipEventSynchronize(b->ctx->ev1)) != hipSuccess)
+        return fail_hip(e, "batch render");
+    if (ms) {
+        float f = 0;
+        if ((e = hipEventElapsedTime(&f, b->ctx->ev0, b->ctx->ev1)) != hipSuccess)
+            return fail_hip(e, "hipEventElapsedTime");
+        *ms = f;
+    }
+    return XD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xd_batch_create(xd_context* ctx, xd_batch** out) {
+    if (!ctx || !out) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    auto* b = new xd_batch();
+    b->ctx = ctx;
+    for (int t = 0; t < 5; ++t) b->tuning.tile_w[t] = 8;
+    b->tuning.tiles_per_grab = 1;
+    if ((e = hipMalloc(&b->d_qbeg, sizeof(uint32_t) * (kQueues + 1))) != hipSuccess ||
+        (e = hipMalloc(&b->d_counters, sizeof(uint32_t) * kQueues)) != hipSuccess) {
+        xd_batch_free(b);
+        return fail_hip(e, "batch allocation");
+    }
+    ctx->batches.insert(b);
+    *out = b;
+    return XD_OK;
+}
+
+int xd_batch_set_views(xd_batch* b, const xd_view* views, uint32_t count) {
+    if (!b || (!views && count)) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    if (count > kMaxBatchViews) return fail(XD_ERR_INVALID_ARGUMENT, "%u views exceed the limit of %u", count, kMaxBatchViews);
+    for (uint32_t i = 0; i < count; ++i) {
+        const xd_view& v = views[i];
+        if (!v.volume || !v.frame) return fail(XD_ERR_INVALID_ARGUMENT, "view %u: null volume or frame", i);
+        if (v.volume->ctx != b->ctx || v.frame->ctx != b->ctx)
+            return fail(XD_ERR_INVALID_ARGUMENT, "view %u: volume or frame belongs to another context", i);
+        int rc = check_traversal(v.traversal, v.volume);
+        if (rc) return rc;
+        if ((rc = check_view(&v.uniforms))) return rc;
+        if (v.frame->w != v.uniforms.output_region.extent[0] || v.frame->h != v.uniforms.output_region.extent[1])
+            return fail(XD_ERR_INVALID_ARGUMENT, "view %u: frame is %ux%u but the output region is %ux%u", i,
+                        v.frame->w, v.frame->h, v.uniforms.output_region.extent[0], v.uniforms.output_region.extent[1]);
+    }
+    hipError_t e = hipSetDevice(b->ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    b->views.assign(views, views + count);
+    return prepare(b);
+}
+
+int xd_batch_set_tuning(xd_batch* b, const xd_batch_tuning* t) {
+    if (!b || !t) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    for (int i = 0; i < 5; ++i)
+        if (t->tile_w[i] != 8 && t->tile_w[i] != 16 && t->tile_w[i] != 32)
+            return fail(XD_ERR_INVALID_ARGUMENT, "tile width %u is not 8, 16 or 32", t->tile_w[i]);
+    if (t->tiles_per_grab < 1 || t->tiles_per_grab > 8)
+        return fail(XD_ERR_INVALID_ARGUMENT, "tiles_per_grab must be 1..8");
+    if (t->waves_per_simd != 0 && t->waves_per_simd != 8)
+        return fail(XD_ERR_INVALID_ARGUMENT, "waves_per_simd must be 0 (compiler's choice) or 8");
+    b->tuning = *t;
+    if (b->views.empty()) return XD_OK;
+    hipError_t e = hipSetDevice(b->ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    return prepare(b);
+}
+
+int xd_batch_get_tuning(const xd_batch* b, xd_batch_tuning* t) {
+    if (!b || !t) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    *t = b->tuning;
+    return XD_OK;
+}
+
+int xd_batch_render(xd_batch* b, double* ms, uint64_t* rays) {
+    if (!b) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    if (rays) *rays = b->total_rays;
+    if (b->views.empty()) {
+        if (ms) *ms = 0;
+        return XD_OK;
+    }
+    hipError_t e = hipSetDevice(b->ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    return run(b, ms);
+}
+
+int xd_batch_autotune(xd_batch* b, uint32_t trials, xd_batch_tuning* chosen) {
+    if (!b) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    if (trials == 0) trials = 3;
+    hipError_t e = hipSetDevice(b->ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    const std::vector<xd_view> all = b->views;
+    xd_batch_tuning best = b->tuning;
+    // tile shape per traversal, timed on that traversal's views alone
+    for (int t = 0; t < 5; ++t) {
+        std::vector<xd_view> sub;
+        for (const auto& v : all)
+            if (v.traversal == t) sub.push_back(v);
+        if (sub.empty()) continue;
+        double best_ms = 1e30;
+        for (uint32_t tw : {8u, 16u, 32u}) {
+            b->views = sub;
+            b->tuning.tile_w[t] = tw;
+            int rc = prepare(b);
+            if (rc) return rc;
+            double m = 1e30;
+            for (uint32_t k = 0; k < trials; ++k) {
+                double ms = 0;
+                if ((rc = run(b, &ms))) return rc;
+                m = std::min(m, ms);
+            }
+            if (m < best_ms) {
+                best_ms = m;
+                best.tile_w[t] = tw;
+            }
+        }
+        b->tuning.tile_w[t] = best.tile_w[t];
+    }
+    // tiles taken per queue grab, on the whole batch
+    b->views = all;
+    double best_ms = 1e30;
+    for (uint32_t g : {1u, 2u, 4u}) {
+        b->tuning.tiles_per_grab = g;
+        int rc = prepare(b);
+        if (rc) return rc;
+        double m = 1e30;
+        for (uint32_t k = 0; k < trials; ++k) {
+            double ms = 0;
+            if ((rc = run(b, &ms))) return rc;
+            m = std::min(m, ms);
+        }
+        if (m < best_ms) {
+            best_ms = m;
+            best.tiles_per_grab = g;
+        }
+    }
+    b->tuning.tiles_per_grab = best.tiles_per_grab;
+    // register budget: compiler's choice or 8 waves per SIMD
+    best_ms = 1e30;
+    for (uint32_t w : {0u, 8u}) {
+        b->tuning.waves_per_simd = w;
+        int rc = prepare(b);
+        if (rc) return rc;
+        double m = 1e30;
+        for (uint32_t k = 0; k < trials; ++k) {
+            double ms = 0;
+            if ((rc = run(b, &ms))) return rc;
+            m = std::min(m, ms);
+        }
+        if (m < best_ms) {
+            best_ms = m;
+            best.waves_per_simd = w;
+        }
+    }
+    b->tuning = best;
+    int rc = prepare(b);
+    if (rc) return rc;
+    if (chosen) *chosen = best;
+    return XD_OK;
+}
+
+int xd_batch_free(xd_batch* b) {
+    if (!b) return XD_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    drop_graph(b);
+    if (b->d_views) (void)hipFree(b->d_views);
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->d_qbeg) (void)hipFree(b->d_qbeg);
+    if (b->d_counters) (void)hipFree(b->d_counters);
+    b->ctx->batches.erase(b);
+    delete b;
+    return XD_OK;
+}
+
+}  // extern "C"

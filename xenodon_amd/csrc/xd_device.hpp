@@ -27,9 +27,14 @@ __device__ __forceinline__ f3 divv(f3 a, f3 b) { return f3{a.x / b.x, a.y / b.y,
 __device__ __forceinline__ f3 muls(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
 __device__ __forceinline__ f3 smul(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
 __device__ __forceinline__ f3 neg(f3 a) { return f3{-a.x, -a.y, -a.z}; }
-// GLSL: min(x, y) = y < x ? y : x; max(x, y) = x < y ? y : x
-__device__ __forceinline__ float gmin(float x, float y) { return y < x ? y : x; }
-__device__ __forceinline__ float gmax(float x, float y) { return x < y ? y : x; }
+// GLSL: min(x, y) = y < x ? y : x; max(x, y) = x < y ? y : x.  For finite
+// operands the hardware v_min_f32 / v_max_f32 return the same value; they
+// can differ only in the sign of a zero result (min(-0, +0)), and no signed
+// zero produced here reaches a division or changes a stored pixel (t values
+// only feed comparisons, subtractions and products that are summed into a
+// non-negative total, and -0 and +0 both store as 0).
+__device__ __forceinline__ float gmin(float x, float y) { return __builtin_fminf(x, y); }
+__device__ __forceinline__ float gmax(float x, float y) { return __builtin_fmaxf(x, y); }
 __device__ __forceinline__ f3 vmin(f3 a, f3 b) { return f3{gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.z)}; }
 __device__ __forceinline__ f3 vmax(f3 a, f3 b) { return f3{gmax(a.x, b.x), gmax(a.y, b.y), gmax(a.z, b.z)}; }
 // common.glsl:58-64

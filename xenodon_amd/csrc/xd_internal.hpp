@@ -14,6 +14,17 @@ constexpr uint32_t kDepthMask = 0x7FFFFFFFu;
 constexpr uint32_t kMaxLevels = 64;  // find() descent cap (cyclic-tree guard)
 constexpr uint32_t kBrick = 4;       // grid bricks are 4x4x4 texels (256 B)
 constexpr uint32_t kBlockThreads = 256;
+// LDS stacks: every wavefront owns slots x 64 entries per array, lane-strided
+constexpr uint32_t kWaveLanes = 64;
+// tagged child words of the derived octree layout
+constexpr uint32_t kTagLeaf = 0x80000000u;   // the child is a leaf
+constexpr uint32_t kTagEmpty = 0x40000000u;  // ... whose RGB is zero
+constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
+constexpr uint64_t kMaxTaggedNodes = 1ull << 30;
+// top tree levels numbered breadth-first and staged in LDS: levels 0..3 hold
+// at most 1 + 8 + 64 + 512 = 585 nodes (18.7 KB of child words)
+constexpr uint32_t kTopLevels = 4;
+constexpr uint32_t kMaxTopNodes = 585;
 
 // Everything one traversal launch needs; passed by value as the kernel
 // argument (kernarg segment), so a launch needs no device-side setup.
@@ -22,15 +33,29 @@ struct RenderParams {
     View view;
     uint32_t* out;
     uint32_t out_pitch;
-    uint32_t stack_slots;  // LDS stack entries per ray (svo-df, esvo)
+    uint32_t stack_slots;  // LDS stack entries this tree needs per ray (svo-df, esvo)
+    uint32_t lds_slots;    // stack entries per ray in the launch's LDS layout (>= stack_slots)
     uint64_t cap;          // iteration cap (never reached on well-formed input)
-    // grid volume (dda): 4x4x4 bricks of packed RGBA8 + one occupancy byte per brick
+    // grid volume (dda): 4x4x4 bricks of packed RGBA8, one occupancy byte per
+    // brick, and a bitmask of non-empty macro cells (2^macro_shift voxels a
+    // side) that the kernel keeps in LDS
     const uint32_t* bricks;
     const uint8_t* occ;
+    const uint32_t* macro;
     uint32_t dims[3];
     uint32_t nb[3];
-    // octree volume: node_count * 10 u32 words
+    uint32_t mn[3];
+    uint32_t macro_shift;
+    uint32_t macro_words;
+    uint32_t wide;  // padded texel count >= 2^32: 64-bit brick addressing
+    // octree volume: the .svo words (node_count * 10), used only for trees
+    // too large to tag; or the tagged layout: child words + {color, depth}
     const uint32_t* nodes;
+    const uint32_t* cw;
+    const uint2* meta;
+    uint32_t root_tag;
+    uint32_t node_count;
+    uint32_t top_nodes;  // child words of nodes [0, top_nodes) are staged in LDS
     unsigned long long* words;  // word counter (instrumented variant only)
 };
 
@@ -38,8 +63,15 @@ struct RenderParams {
 // selects the per-wavefront pixel tile (8x8, 16x4, 32x2).
 hipError_t launch_traversal(int traversal, int tile_w, bool count_words, const RenderParams& p, hipStream_t s);
 
-// Bytes of dynamic LDS a traversal needs per 256-thread block.
-size_t traversal_lds_bytes(int traversal, uint32_t stack_slots);
+// Bytes of dynamic LDS a traversal needs per block of `threads` threads.
+size_t traversal_lds_bytes(int traversal, uint32_t stack_slots, uint32_t top_nodes, uint32_t threads);
+
+// Largest macro-cell bitmask the DDA kernel stages in LDS (16 KiB).
+constexpr uint32_t kMaxMacroWords = 4096;
+
+// Build the macro-cell occupancy bitmask from the brick occupancy bytes.
+hipError_t launch_macro_mask(const uint8_t* occ, const uint32_t nb[3], uint32_t macro_shift, const uint32_t mn[3],
+                             uint32_t* macro, uint32_t words, hipStream_t s);
 
 // Grid upload: rewrite `layers` z-layers (X*Y texels each, x fastest) held in
 // `staging` into bricks of brick-layer `bz`, and set the occupancy bytes.

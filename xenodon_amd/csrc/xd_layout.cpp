@@ -1,0 +1,96 @@
+// Device layout of an octree: tagged child words in sibling-grouped order.
+//
+// The .svo numbering (reverse post-order, OctreeConstruction.h:92-112)
+// scatters the eight children of a node across its subtrees.  On upload the
+// nodes are renumbered so that the children of every internal node take eight
+// consecutive indices, sibling groups being allocated depth-first from the
+// root (node 0 stays node 0, which the rope traversal uses as "no
+// neighbour").  Rays that pass through one parent then read child words
+// from the same two cache lines, and a subtree occupies one contiguous range.
+// In a DAG a shared node keeps the index of its first visit.  The first
+// kTopLevels levels are numbered breadth-first ahead of everything else, so
+// their child words form one small block that the kernels stage in LDS.
+//
+// child_w[i*8 + k] = new index of child k | kTagLeaf (child is a leaf)
+//                    | kTagEmpty (child is a leaf with zero RGB);
+// leaves' child words are their ropes, renumbered and tagged the same way.
+// meta[i] = {color, is_leaf_depth} of node i.
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "xd_runtime.hpp"
+
+namespace xd {
+
+void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels]) {
+    std::vector<uint32_t> new_id(n, 0xFFFFFFFFu);
+    uint32_t next = 1;
+    new_id[0] = 0;
+    // levels 0..kTopLevels-1 in breadth-first order: the lowest indices
+    std::vector<uint32_t> level{0};
+    top[0] = 1;
+    for (uint32_t L = 1; L < kTopLevels; ++L) {
+        std::vector<uint32_t> nxt;
+        for (uint32_t old : level) {
+            const uint32_t* nd = w + (uint64_t)old * kNodeWords;
+            if (nd[9] & kLeafMask) continue;
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t c = nd[k];
+                if (new_id[c] == 0xFFFFFFFFu) {
+                    new_id[c] = next++;
+                    nxt.push_back(c);
+                }
+            }
+        }
+        level.swap(nxt);
+        top[L] = next;  // nodes of levels 0..L
+    }
+    // below: sibling groups, depth-first from each node of the last top level
+    std::vector<uint32_t> stack;
+    uint32_t fresh[8];
+    for (auto it = level.rbegin(); it != level.rend(); ++it) stack.push_back(*it);
+    while (!stack.empty()) {
+        const uint32_t old = stack.back();
+        stack.pop_back();
+        const uint32_t* nd = w + (uint64_t)old * kNodeWords;
+        if (nd[9] & kLeafMask) continue;
+        int nf = 0;
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t c = nd[k];
+            if (new_id[c] == 0xFFFFFFFFu) {
+                new_id[c] = next++;
+                fresh[nf++] = c;
+            }
+        }
+        // visit child 0's group first
+        for (int k = nf - 1; k >= 0; --k) stack.push_back(fresh[k]);
+    }
+    // nodes no internal node points to (never traversed) go last
+    for (uint64_t i = 0; i < n; ++i)
+        if (new_id[i] == 0xFFFFFFFFu) new_id[i] = next++;
+
+    const unsigned T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; ++t)
+        th.emplace_back([&, t] {
+            for (uint64_t i = t; i < n; i += T) {
+                const uint32_t* nd = w + i * kNodeWords;
+                const uint32_t ni = new_id[i];
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t c = nd[k];
+                    const uint32_t* cn = w + (uint64_t)c * kNodeWords;
+                    uint32_t tag = 0;
+                    if (cn[9] & kLeafMask) tag = kTagLeaf | ((cn[8] & 0xFFFFFFu) ? 0u : kTagEmpty);
+                    child_w[(uint64_t)ni * 8 + k] = new_id[c] | tag;
+                }
+                meta[(uint64_t)ni * 2] = nd[8];
+                meta[(uint64_t)ni * 2 + 1] = nd[9];
+            }
+        });
+    for (auto& x : th) x.join();
+}
+
+}  // namespace xd

@@ -90,13 +90,25 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
     if (vol->kind == XD_VOLUME_GRID) {
         p.bricks = vol->d_bricks;
         p.occ = vol->d_occ;
+        p.macro = vol->d_macro;
+        p.macro_shift = vol->macro_shift;
+        p.macro_words = vol->macro_words;
+        p.wide = (uint64_t)vol->nb[0] * vol->nb[1] * vol->nb[2] * 64 > 0xFFFFFFFFull ? 1u : 0u;
         for (int i = 0; i < 3; ++i) {
             p.dims[i] = (uint32_t)vol->dims[i];
             p.nb[i] = vol->nb[i];
+            p.mn[i] = vol->mn[i];
         }
     } else {
         p.nodes = vol->d_nodes;
+        p.cw = vol->d_cw;
+        p.meta = vol->d_meta;
+        p.root_tag = vol->root_tag;
+        // levels 0..3 for the stackless traversals, 0..2 next to an LDS stack
+        p.top_nodes = vol->d_cw ? vol->top[(traversal == XD_SVO_NAIVE || traversal == XD_SVO_ROPE) ? 3 : 2] : 0;
         p.stack_slots = vol->max_depth + 1;
+        p.lds_slots = p.stack_slots;
+        p.node_count = (uint32_t)vol->node_count;
     }
 }
 
@@ -176,11 +188,24 @@ int xd_context_set_tile(xd_context* ctx, uint32_t tile_w) {
     return XD_OK;
 }
 
+int xd_context_set_option(xd_context* ctx, const char* name, int64_t value) {
+    if (!ctx || !name) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    if (strcmp(name, "octree_layout") == 0) {
+        if (value != 0 && value != 1) return fail(XD_ERR_INVALID_ARGUMENT, "octree_layout must be 0 or 1");
+        ctx->raw_octrees = (int)value;
+        return XD_OK;
+    }
+    if (strcmp(name, "tile_w") == 0) return xd_context_set_tile(ctx, (uint32_t)value);
+    return fail(XD_ERR_INVALID_ARGUMENT, "unknown option '%s'", name);
+}
+
 int xd_close(xd_context* ctx) {
     if (!ctx) return XD_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    // volumes and frames still alive are released with their context
+    // batches, volumes and frames still alive are released with their context
+    std::vector<xd_batch*> bats(ctx->batches.begin(), ctx->batches.end());
+    for (auto* b : bats) xd_batch_free(b);
     std::vector<xd_volume*> vols(ctx->volumes.begin(), ctx->volumes.end());
     for (auto* v : vols) xd_volume_free(v);
     std::vector<xd_frame*> frs(ctx->frames.begin(), ctx->frames.end());
@@ -237,6 +262,25 @@ int xd_volume_from_grid(xd_context* ctx, const void* rgba8, uint64_t dim_x, uint
         }
     }
     (void)hipFree(staging);
+    // macro cells: the smallest power-of-two side >= 8 voxels whose bitmask
+    // fits the kernel's LDS staging area
+    for (v->macro_shift = 3;; ++v->macro_shift) {
+        uint64_t cells = 1;
+        for (int i = 0; i < 3; ++i) {
+            v->mn[i] = (uint32_t)((v->dims[i] + (1ull << v->macro_shift) - 1) >> v->macro_shift);
+            cells *= v->mn[i];
+        }
+        v->macro_words = (uint32_t)((cells + 31) / 32);
+        if (v->macro_words <= kMaxMacroWords) break;
+    }
+    if ((e = hipMalloc(&v->d_macro, (size_t)v->macro_words * 4)) != hipSuccess ||
+        (e = launch_macro_mask(v->d_occ, v->nb, v->macro_shift, v->mn, v->d_macro, v->macro_words, ctx->stream)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(ctx->stream)) != hipSuccess) {
+        xd_volume_free(v);
+        return fail_hip(e, "grid occupancy");
+    }
+    v->bytes += (uint64_t)v->macro_words * 4;
     ctx->volumes.insert(v);
     clear_error();
     *out = v;
@@ -250,6 +294,9 @@ static int analyse_octree(const uint32_t* w, uint64_t n, uint32_t* max_depth, ui
     for (uint64_t i = 0; i < n; ++i) {
         const uint32_t* nd = w + i * kNodeWords;
         const bool leaf = (nd[9] & kLeafMask) != 0;
+        if ((nd[9] & kDepthMask) > 64)
+            return fail(XD_ERR_FORMAT, "node %llu has depth field %u (above 64)", (unsigned long long)i,
+                        nd[9] & kDepthMask);
         for (int k = 0; k < 8; ++k) {
             if (nd[k] >= n)
                 return fail(XD_ERR_FORMAT, "node %llu child %d = %u is out of range (%llu nodes)",
@@ -322,14 +369,34 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
     v->max_depth = max_depth;
     v->is_rope = is_rope;
     const uint64_t bytes = node_count * kNodeWords * sizeof(uint32_t);
-    if ((e = hipMalloc(&v->d_nodes, bytes)) != hipSuccess) {
-        xd_volume_free(v);
-        return fail_hip(e, "octree allocation");
-    }
     v->bytes = bytes;
-    if ((e = hipMemcpy(v->d_nodes, nodes, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
-        xd_volume_free(v);
-        return fail_hip(e, "octree upload");
+    if (node_count <= kMaxTaggedNodes && !ctx->raw_octrees) {
+        // tagged, sibling-grouped layout, derived on the host
+        std::vector<uint32_t> cw, meta;
+        try {
+            cw.resize(node_count * 8);
+            meta.resize(node_count * 2);
+        } catch (const std::bad_alloc&) {
+            xd_volume_free(v);
+            return fail(XD_ERR_OUT_OF_MEMORY, "out of host memory deriving the octree layout");
+        }
+        build_tagged_layout(w, node_count, cw.data(), meta.data(), v->top);
+        if ((e = hipMalloc(&v->d_cw, node_count * 8 * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipMalloc(&v->d_meta, node_count * sizeof(uint2))) != hipSuccess ||
+            (e = hipMemcpy(v->d_cw, cw.data(), node_count * 8 * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
+                hipSuccess ||
+            (e = hipMemcpy(v->d_meta, meta.data(), node_count * sizeof(uint2), hipMemcpyHostToDevice)) != hipSuccess) {
+            xd_volume_free(v);
+            return fail_hip(e, "octree upload");
+        }
+        const uint32_t root_ld = w[9], root_col = w[8];
+        v->root_tag = (root_ld & kLeafMask) ? (kTagLeaf | ((root_col & 0xFFFFFFu) ? 0u : kTagEmpty)) : 0u;
+    } else {
+        if ((e = hipMalloc(&v->d_nodes, bytes)) != hipSuccess ||
+            (e = hipMemcpy(v->d_nodes, nodes, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+            xd_volume_free(v);
+            return fail_hip(e, "octree upload");
+        }
     }
     ctx->volumes.insert(v);
     clear_error();
@@ -356,7 +423,10 @@ int xd_volume_free(xd_volume* v) {
     (void)hipStreamSynchronize(v->ctx->stream);
     if (v->d_bricks) (void)hipFree(v->d_bricks);
     if (v->d_occ) (void)hipFree(v->d_occ);
+    if (v->d_macro) (void)hipFree(v->d_macro);
     if (v->d_nodes) (void)hipFree(v->d_nodes);
+    if (v->d_cw) (void)hipFree(v->d_cw);
+    if (v->d_meta) (void)hipFree(v->d_meta);
     v->ctx->volumes.erase(v);
     delete v;
     return XD_OK;

@@ -12,12 +12,14 @@
 struct xd_context {
     int device = 0;
     int tile_w = 8;  // per-wavefront tile width: 8 (8x8), 16 (16x4), 32 (32x2)
+    int raw_octrees = 0;  // keep .svo words instead of the tagged layout
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long* d_words = nullptr;
     // live objects created on this context; xd_close frees what is left
     std::unordered_set<struct xd_volume*> volumes;
     std::unordered_set<struct xd_frame*> frames;
+    std::unordered_set<struct xd_batch*> batches;
 };
 
 struct xd_volume {
@@ -33,8 +35,17 @@ struct xd_volume {
     uint32_t* d_bricks = nullptr;
     uint8_t* d_occ = nullptr;
     uint32_t nb[3] = {0, 0, 0};
-    // octree: node_count * 10 words, the .svo node layout
+    uint32_t* d_macro = nullptr;  // non-empty macro cells, 1 bit each
+    uint32_t mn[3] = {0, 0, 0};
+    uint32_t macro_shift = 0;
+    uint32_t macro_words = 0;
+    // octree: the tagged layout (child words + {color, depth}) when the tree
+    // has at most 2^30 nodes, else the .svo words as they are
     uint32_t* d_nodes = nullptr;
+    uint32_t* d_cw = nullptr;
+    uint2* d_meta = nullptr;
+    uint32_t root_tag = 0;
+    uint32_t top[xd::kTopLevels] = {0, 0, 0, 0};  // nodes in breadth-first levels 0..L
 };
 
 struct xd_frame {
@@ -53,5 +64,9 @@ int check_traversal(int traversal, const xd_volume* vol);
 int check_view(const xd_uniforms* u);
 uint64_t iteration_cap(int traversal, const xd_volume* vol, const uint32_t model_dim[3]);
 void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_camera* cam, const xd_uniforms* u);
+// Host-side derivation of the tagged, sibling-grouped device layout
+// (child_w: n*8 words, meta: n*2 words) from .svo node words.
+// top[L] = number of nodes in levels 0..L, which hold indices [0, top[L]).
+void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels]);
 
 }  // namespace xd

@@ -1,0 +1,352 @@
+// Octree traversals on the derived device layout (tagged child words).
+//
+// Layout: child_w[n*8 + k] = index of child k | kTagLeaf if that child is a
+// leaf | kTagEmpty if it is a leaf whose RGB is zero; meta[n] = {color,
+// is_leaf_depth}.  A leaf's child words are its ropes, tagged the same way.
+// The traversal arithmetic is the shaders' (see xd_svo_ref.hpp); only loads
+// change:
+//   * "is this node a leaf?" comes with the pointer that led to it, so a
+//     descent costs one dependent load per level instead of two;
+//   * the colour of an empty leaf is known to be 0 without a load (the sum
+//     still adds 0 * step, exactly as the shader does);
+//   * svo-df fetches a node's eight child words with two 16-byte loads and
+//     then steps through its children from registers.
+#pragma once
+
+#include "xd_svo_ref.hpp"
+
+namespace xd {
+
+struct FastSvo {
+    const uint32_t* cw;
+    const uint2* meta;
+    uint32_t root_tag;     // kTagLeaf / kTagEmpty of node 0
+    const uint32_t* top;   // LDS copy of the child words of nodes [0, n_top)
+    uint32_t n_top;
+    uint32_t n;            // node count (indices read back from a stack are clamped to it)
+};
+
+// Child word k of node n: from LDS for the breadth-first top levels.
+__device__ __forceinline__ uint32_t cw_at(const FastSvo& s, uint32_t n, uint32_t k) {
+    if (n < s.n_top) return *(const __attribute__((address_space(3))) uint32_t*)(s.top + n * 8 + k);
+    return *(const __attribute__((address_space(1))) uint32_t*)(s.cw + (uint64_t)n * 8 + k);
+}
+
+// GLSL mod(x, y) for y = 2^k: x / y == x * 2^-k exactly (no rounding in
+// either), so the division is a multiplication by the exact reciprocal.
+__device__ __forceinline__ float pmod(float x, float y, float inv_y) { return x - y * __builtin_floorf(x * inv_y); }
+
+__device__ __forceinline__ uint32_t colour_if(const FastSvo& s, uint32_t node, uint32_t tag) {
+    return (tag & kTagEmpty) ? 0u : s.meta[node].x;
+}
+
+// svo_naive.comp:8-27 with tagged pointers.  `tag` returns the found node's tags.
+__device__ __forceinline__ uint32_t fast_find(const FastSvo& s, f3 pos, f3& base, float& side, uint32_t& tag) {
+    float extent = 1.0f;
+    uint32_t index = 0;
+    f3 offset = splat(0.0f);
+    tag = s.root_tag;
+    for (uint32_t level = 0;; ++level) {
+        if ((tag & kTagLeaf) || level > kMaxLevels) {
+            base = offset;
+            side = extent;
+            return index;
+        }
+        extent = extent * 0.5f;
+        bool mx = pos.x >= offset.x + extent;
+        bool my = pos.y >= offset.y + extent;
+        bool mz = pos.z >= offset.z + extent;
+        uint32_t child = (mx ? 4u : 0u) + (my ? 2u : 0u) + (mz ? 1u : 0u);
+        offset = add(offset, mul(mk(mx ? 1.0f : 0.0f, my ? 1.0f : 0.0f, mz ? 1.0f : 0.0f), splat(extent)));
+        const uint32_t wd = cw_at(s, index, child);
+        index = wd & kIndexMask;
+        tag = wd;
+    }
+}
+
+// svo_naive.comp:29-71
+__device__ __forceinline__ f3 fast_naive(const FastSvo& s, uint64_t cap, f3 ro, f3 rd) {
+    const float min_step = 0.00001f;
+    f3 rrd = divv(splat(1.0f), rd);
+    f3 bias = mul(rrd, ro);
+    f3 box_min = neg(bias);
+    f3 box_max = sub(rrd, bias);
+    float t_min = max_elem(vmin(box_min, box_max));
+    float t_max = min_elem(vmax(box_min, box_max));
+    if (t_min > t_max) return splat(0.0f);
+    t_min = gmax(t_min, 0.0f);
+    f3 total = splat(0.0f);
+    float t = t_min + min_step;
+    uint64_t it = 0;
+    while (t < t_max) {
+        if (++it > cap) break;
+        f3 pt = add(smul(t, rd), ro);
+        f3 offset;
+        float side;
+        uint32_t tag;
+        uint32_t node = fast_find(s, pt, offset, side, tag);
+        f3 node_min = sub(mul(offset, rrd), bias);
+        f3 node_max = sub(mul(add(offset, splat(side)), rrd), bias);
+        float u_min = max_elem(vmin(node_min, node_max));
+        float u_max = min_elem(vmax(node_min, node_max));
+        u_min = gmax(u_min, 0.0f);
+        float step = gmax(u_max - u_min, min_step);
+        t = t + step;
+        total = add(total, muls(unorm_rgb(colour_if(s, node, tag)), step));
+    }
+    return total;
+}
+
+__device__ __forceinline__ uint32_t pick8(const uint4& lo, const uint4& hi, uint32_t ci) {
+    const uint32_t a = (ci & 1u) ? lo.y : lo.x, b = (ci & 1u) ? lo.w : lo.z;
+    const uint32_t c = (ci & 1u) ? hi.y : hi.x, d = (ci & 1u) ? hi.w : hi.z;
+    const uint32_t e = (ci & 2u) ? b : a, f = (ci & 2u) ? d : c;
+    return (ci & 4u) ? f : e;
+}
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint4 to_uint4(u32x4 v) { return make_uint4(v.x, v.y, v.z, v.w); }
+
+__device__ __forceinline__ void load8(const FastSvo& s, uint32_t node, uint4& lo, uint4& hi) {
+    // LDS and global loads kept in their own address spaces (a pointer
+    // selected between them would become a slower flat load)
+    if (node < s.n_top) {
+        const __attribute__((address_space(3))) u32x4* p =
+            (const __attribute__((address_space(3))) u32x4*)(s.top + node * 8);
+        lo = to_uint4(p[0]);
+        hi = to_uint4(p[1]);
+    } else {
+        const __attribute__((address_space(1))) u32x4* p =
+            (const __attribute__((address_space(1))) u32x4*)(s.cw + (uint64_t)node * 8);
+        lo = to_uint4(p[0]);
+        hi = to_uint4(p[1]);
+    }
+}
+
+// Slab values of the three planes per axis shared by a node's eight child
+// boxes: p0 = corner, p1 = corner + side, p2 = p1 + side; T = p * rrd - bias.
+// Child box k has, per axis, box_min = T[b] and box_max = T[b + 1] with b the
+// axis bit of k -- the same floats svo_df.comp computes from pos and pos+side.
+struct Planes {
+    float x0, x1, x2, y0, y1, y2, z0, z1, z2;
+};
+
+__device__ __forceinline__ Planes node_planes(f3 corner, float side, f3 rrd, f3 bias) {
+    Planes P;
+    const float px1 = corner.x + side, py1 = corner.y + side, pz1 = corner.z + side;
+    P.x0 = corner.x * rrd.x - bias.x;
+    P.x1 = px1 * rrd.x - bias.x;
+    P.x2 = (px1 + side) * rrd.x - bias.x;
+    P.y0 = corner.y * rrd.y - bias.y;
+    P.y1 = py1 * rrd.y - bias.y;
+    P.y2 = (py1 + side) * rrd.y - bias.y;
+    P.z0 = corner.z * rrd.z - bias.z;
+    P.z1 = pz1 * rrd.z - bias.z;
+    P.z2 = (pz1 + side) * rrd.z - bias.z;
+    return P;
+}
+
+// svo_df.comp:6-66.  The shader's walk (children in index order, stack of
+// (node, child index), tail call for child 7) with the per-child box taken
+// from the node's planes.  The child position the shader carries in `pos`
+// is recomputed where it is needed: on descent (corner + side on the set
+// axes) and on pop (the shader's pos - mod(pos, 2 * side) from the position
+// of child 7, corner + side).
+__device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t slots, f3 ro, f3 rd,
+                                      uint32_t* stk_node, uint32_t* stk_child) {
+    f3 rrd = divv(splat(1.0f), rd);
+    f3 bias = mul(rrd, ro);
+    const int cap_sp = (int)slots;
+    int sp = 0;
+    uint32_t node = 0, child_idx = 0;
+    f3 corner = splat(0.0f);
+    float side = 0.5f;
+    f3 total = splat(0.0f);
+    uint4 lo, hi;
+    load8(s, node, lo, hi);
+    Planes P = node_planes(corner, side, rrd, bias);
+    const uint32_t cap32 = cap > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)cap;
+    uint32_t it = 0;
+    for (;;) {
+        if (++it > cap32) break;
+        const uint32_t wd = pick8(lo, hi, child_idx);
+        const bool bx = (child_idx & 4u) != 0, by = (child_idx & 2u) != 0, bz = (child_idx & 1u) != 0;
+        const float mnx = bx ? P.x1 : P.x0, mxx = bx ? P.x2 : P.x1;
+        const float mny = by ? P.y1 : P.y0, mxy = by ? P.y2 : P.y1;
+        const float mnz = bz ? P.z1 : P.z0, mxz = bz ? P.z2 : P.z1;
+        const float t_min = max_elem(mk(gmin(mnx, mxx), gmin(mny, mxy), gmin(mnz, mxz)));
+        const float t_max = min_elem(mk(gmax(mnx, mxx), gmax(mny, mxy), gmax(mnz, mxz)));
+        if (t_min < t_max && t_max > 0.0f) {
+            const uint32_t child = wd & kIndexMask;
+            if (wd & kTagLeaf) {
+                total = add(total, muls(unorm_rgb(colour_if(s, child, wd)), t_max - gmax(t_min, 0.0f)));
+            } else if (child_idx == 7 || sp < cap_sp) {
+                if (child_idx != 7) {
+                    stk_node[sp * kWaveLanes] = node;
+                    stk_child[sp * kWaveLanes] = child_idx;
+                    ++sp;
+                }
+                corner = mk(bx ? corner.x + side : corner.x, by ? corner.y + side : corner.y,
+                            bz ? corner.z + side : corner.z);
+                side = side * 0.5f;
+                node = child;
+                child_idx = 0;
+                load8(s, node, lo, hi);
+                P = node_planes(corner, side, rrd, bias);
+                continue;
+            }
+        }
+        if (child_idx == 7) {
+            --sp;
+            if (sp < 0) break;
+            node = min(stk_node[sp * kWaveLanes], s.n - 1u);
+            child_idx = stk_child[sp * kWaveLanes] & 7u;
+            load8(s, node, lo, hi);
+            const int d = (int)(s.meta[node].y & kDepthMask);
+            const f3 pos = mk(corner.x + side, corner.y + side, corner.z + side);  // child 7
+            side = __builtin_ldexpf(1.0f, -d) * 0.5f;
+            // side is a power of two, so mod(pos, 2 side) is exact with the reciprocal
+            const float s2 = side * 2.0f;
+            const float inv = __builtin_ldexpf(1.0f, -__builtin_amdgcn_frexp_expf(s2) + 1);
+            corner = sub(pos, mk(pmod(pos.x, s2, inv), pmod(pos.y, s2, inv), pmod(pos.z, s2, inv)));
+            P = node_planes(corner, side, rrd, bias);
+        }
+        ++child_idx;
+    }
+    return total;
+}
+
+// svo_rope.comp:65-135 (find_relative inlined)
+__device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f3 rd) {
+    f3 sgn = mk(fsign(rd.x), fsign(rd.y), fsign(rd.z));
+    const uint32_t nbx = 1u - (sgn.x > 0.0f ? 1u : 0u);
+    const uint32_t nby = 3u - (sgn.y > 0.0f ? 1u : 0u);
+    const uint32_t nbz = 5u - (sgn.z > 0.0f ? 1u : 0u);
+    sgn = add(sgn, splat(0.1f));
+    f3 rrd = divv(splat(1.0f), rd);
+    f3 bias = mul(rrd, ro);
+    f3 box_min = neg(bias);
+    f3 box_max = sub(rrd, bias);
+    float t_min = max_elem(vmin(box_min, box_max));
+    float t_max = min_elem(vmax(box_min, box_max));
+    if (t_min > t_max) return splat(0.0f);
+    t_min = gmax(t_min, 0.0f);
+    f3 pos = add(ro, smul(t_min, rd));
+    f3 offset;
+    float side;
+    uint32_t tag;
+    uint32_t node = fast_find(s, pos, offset, side, tag);
+    f3 node_min = sub(mul(offset, rrd), bias);
+    f3 node_max = sub(mul(add(offset, splat(side)), rrd), bias);
+    f3 far = vmax(node_min, node_max);
+    float u_min = max_elem(vmin(node_min, node_max));
+    float u_max = min_elem(far);
+    float step = u_max - gmax(u_min, 0.0f);
+    f3 total = muls(unorm_rgb(colour_if(s, node, tag)), step);
+    f3 mask;
+    uint32_t n = neighbor_index(nbx, nby, nbz, far, mask);
+    uint32_t wd = cw_at(s, node, n);
+    node = wd & kIndexMask;
+    tag = wd;
+    offset = add(offset, muls(mul(mask, sgn), side));
+    uint64_t it = 0;
+    while (node != 0) {
+        if (++it > cap) break;
+        pos = add(ro, smul(u_max, rd));
+        // find_relative (svo_rope.comp:29-48) from the rope's target
+        const uint2 m = s.meta[node];
+        const uint32_t start = node;
+        const int d = (int)(m.y & kDepthMask);
+        float extent = __builtin_ldexpf(1.0f, -d);
+        const float inv = __builtin_ldexpf(1.0f, d);
+        offset = sub(offset, mk(pmod(offset.x, extent, inv), pmod(offset.y, extent, inv), pmod(offset.z, extent, inv)));
+        for (uint32_t level = 0;; ++level) {
+            if ((tag & kTagLeaf) || level > kMaxLevels) break;
+            extent = extent * 0.5f;
+            bool mx = pos.x >= offset.x + extent;
+            bool my = pos.y >= offset.y + extent;
+            bool mz = pos.z >= offset.z + extent;
+            uint32_t child = (mx ? 4u : 0u) + (my ? 2u : 0u) + (mz ? 1u : 0u);
+            offset = add(offset, mul(mk(mx ? 1.0f : 0.0f, my ? 1.0f : 0.0f, mz ? 1.0f : 0.0f), splat(extent)));
+            wd = cw_at(s, node, child);
+            node = wd & kIndexMask;
+            tag = wd;
+        }
+        side = extent;
+        node_min = sub(mul(offset, rrd), bias);
+        node_max = sub(mul(add(offset, splat(side)), rrd), bias);
+        far = vmax(node_min, node_max);
+        u_min = max_elem(vmin(node_min, node_max));
+        u_max = min_elem(far);
+        step = u_max - gmax(u_min, 0.0f);
+        const uint32_t col = (tag & kTagEmpty) ? 0u : (node == start ? m.x : s.meta[node].x);
+        total = add(total, muls(unorm_rgb(col), step));
+        n = neighbor_index(nbx, nby, nbz, far, mask);
+        wd = cw_at(s, node, n);
+        node = wd & kIndexMask;
+        tag = wd;
+        offset = add(offset, muls(mul(mask, sgn), side));
+    }
+    return total;
+}
+
+// esvo.comp:30-136
+__device__ __forceinline__ f3 fast_esvo(const FastSvo& s, uint64_t cap, uint32_t slots, f3 ro, f3 rd,
+                                        uint32_t* stk_parent, float* stk_tmax) {
+    EsvoRay r = esvo_setup(ro, rd);
+    float t_min = r.t_min, t_max = r.t_max, h = r.h;
+    f3 pos = r.pos;
+    uint32_t idx = r.idx, parent = 0, scale = 22;
+    float scale_exp2 = 0.5f;
+    f3 total = splat(0.0f);
+    uint64_t it = 0;
+    while (scale < 23u) {
+        if (++it > cap) break;
+        f3 t_corner = sub(mul(pos, r.t_coeff), r.t_bias);
+        float tc_max = min_elem(t_corner);
+        if (t_min <= t_max) {
+            float tv_max = gmin(t_max, tc_max);
+            if (t_min <= tv_max) {
+                const uint32_t wd = cw_at(s, parent, idx ^ r.octant_mask);
+                const uint32_t child = wd & kIndexMask;
+                if (wd & kTagLeaf) {
+                    total = add(total, muls(unorm_rgb(colour_if(s, child, wd)), tv_max - t_min));
+                } else {
+                    if (tc_max < h) {
+                        uint32_t slot = 22u - scale;
+                        if (slot < slots) {
+                            stk_parent[slot * kWaveLanes] = parent;
+                            stk_tmax[slot * kWaveLanes] = t_max;
+                        }
+                    }
+                    h = tc_max;
+                    parent = child;
+                    --scale;
+                    scale_exp2 = scale_exp2 * 0.5f;
+                    f3 t_center = add(smul(scale_exp2, r.t_coeff), t_corner);
+                    bool ax = t_center.x > t_min, ay = t_center.y > t_min, az = t_center.z > t_min;
+                    idx = cxor(0u, ax, ay, az);
+                    pos = add(pos, mk(ax ? scale_exp2 : 0.0f, ay ? scale_exp2 : 0.0f, az ? scale_exp2 : 0.0f));
+                    t_max = tv_max;
+                    continue;
+                }
+            }
+        }
+        bool ax = t_corner.x <= tc_max, ay = t_corner.y <= tc_max, az = t_corner.z <= tc_max;
+        uint32_t step_mask = cxor(0u, ax, ay, az);
+        pos = sub(pos, mk(ax ? scale_exp2 : 0.0f, ay ? scale_exp2 : 0.0f, az ? scale_exp2 : 0.0f));
+        t_min = tc_max;
+        idx ^= step_mask;
+        if ((idx & step_mask) != 0) {
+            if (!esvo_pop(pos, scale_exp2, scale, idx, ax, ay, az)) break;
+            uint32_t slot = 22u - scale;
+            parent = slot < slots ? min(stk_parent[slot * kWaveLanes], s.n - 1u) : 0u;
+            t_max = slot < slots ? stk_tmax[slot * kWaveLanes] : 0.0f;
+            h = 0.0f;
+        }
+    }
+    return total;
+}
+
+}  // namespace xd
