@@ -114,7 +114,7 @@ def main():
     ap.add_argument("--frame-stride", type=int, default=1, help="step k renders camera frame k*stride")
     ap.add_argument("--batch-frames", type=int, default=10, help="camera frames per batched launch")
     ap.add_argument("--per-traversal-groups", type=int, default=3,
-                    help="launches timed per traversal for the per-traversal batched rates")
+                    help="launches timed per traversal for the per-traversal batched rates (0: skip)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--traversals", default=",".join(TRAV_ORDER))
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -234,13 +234,13 @@ def main():
 
     # per-traversal batched rate: groups of one traversal only
     per_trav_batched = {}
-    for name in travs:
+    for name in (travs if args.per_traversal_groups > 0 else []):
         vol, md = vols[name]
         sub = ctx.batch()
         sub.set_tuning(tuning["tile_w"], tuning["tiles_per_grab"], tuning["waves_per_simd"])
         ms_sum = 0.0
         rays_sum = 0
-        n_groups = max(1, min(len(groups), args.per_traversal_groups))
+        n_groups = min(len(groups), args.per_traversal_groups)
         for g in range(n_groups):
             k0 = groups[(g * len(groups)) // n_groups]  # spread over the path
             views = []

@@ -1,6 +1,19 @@
 /*
  * Oracle traversal entry points.  TEST INFRASTRUCTURE ONLY (see xo.h).
  * Build with: gcc -O2 -std=c11 -ffp-contract=off -fno-fast-math -fopenmp
+ *
+ * Parity status (tests/test_oracle_trace.py):
+ *   xo_ray_dir, emission coefficient   pinned: uniform-cube closed form
+ *                                      (pixel = c*e*chord) at every pixel
+ *   DDA (dda.comp:13-53)               pinned: empty/miss closed forms, uniform
+ *                                      cube, FP64 brute-force voxel-box integral
+ *                                      on 2^3..16^3 and ragged 3x7x5 grids
+ *   svo-naive, svo-df, esvo, svo-rope  pinned: the same closed forms and
+ *                                      brute-force integrals on the lossless
+ *                                      tree, cross-traversal invariant
+ *   FP64 twin                          pinned: agrees with FP32 to 2e-5 rel.
+ *   word counters                      pinned: hand-derived known answers
+ *   xo_iteration_cap                   pinned: never reached on well-formed input
  */
 #include "xo.h"
 
