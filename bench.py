@@ -1,24 +1,33 @@
 """Benchmark: Mrays/s of the five traversals at 1920x1080 on the 512x361x512
-CT-like volume over the reference's 150-frame benchmark camera path.
+CT-like volume over the reference's 150-frame benchmark camera path, in the
+plain, batched and served forms.
 
 A step renders one camera frame with each of the five traversals
 (dda on the grid; svo-naive, esvo, svo-df, svo-rope on its lossless rope
-octree), one launch per traversal per frame -- the reference's plain form
-(src/render/Renderer.cpp:55-103).  Inputs are HBM-resident before timing.
+octree).  Inputs are HBM-resident before timing.
+  plain    one launch per traversal per frame -- the reference's form
+           (src/render/Renderer.cpp:55-103); per-traversal Mrays/s use the
+           launches' own HIP-event time (RenderStats semantics,
+           src/render/RenderStats.cpp:46-83).
+  batched  `batch_frames` steps (x 5 traversals) per persistent-kernel
+           hipGraph launch.  `value` = total rays / wall time of the timed
+           batched steps.
+  served   Poisson request arrivals (seeded cameras, log-uniform sizes from
+           128x128 to 1920x1080, traversal uniform over all five) against
+           the resident 128^3 and 512x361x512 volumes; p50/p99 latency from
+           arrival to the frame in host memory, and aggregate Mrays/s.
 
-Prints one JSON line: value = total rays / wall time of the K timed steps;
-per-traversal Mrays/s use the launches' own HIP-event time (the reference's
-RenderStats semantics, src/render/RenderStats.cpp:46-83).  The roofline
-object covers the dominant kernel: algorithmic bytes (4 B per volume word
-the reference algorithm reads, counted by the instrumented kernel variant on
-the same frames, + 4 B per pixel) over its mean launch time.  cpu_baseline
-is the oracle (CPU restatement of the same shaders) on the host cores over a
-bounded band of rows of the same frames.
+The roofline object covers the dominant kernel: algorithmic bytes (4 B per
+volume word the reference algorithm reads, counted by the instrumented kernel
+variant on the same frames, + 4 B per pixel) over its mean launch time.
+cpu_baseline is the oracle (CPU restatement of the same shaders) on the host
+cores over a bounded sample of the same frames.
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -92,6 +101,76 @@ def cpu_baseline(grid, nodes, side, cams, frames, threads, budget_s=30.0):
     }
 
 
+def look_at(eye, target, world_up=(0.0, 1.0, 0.0)):
+    eye, target = np.asarray(eye, np.float64), np.asarray(target, np.float64)
+    f = target - eye
+    f /= np.linalg.norm(f)
+    up = np.asarray(world_up, np.float64)
+    up = up - f * np.dot(up, f)
+    up /= np.linalg.norm(up)
+    return f, up, eye
+
+
+def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0):
+    """Poisson arrivals at `rate` req/s for `duration_s` (or n_saturated
+    requests all released at once).  volumes: name -> (kind, data, model_dim).
+    Returns latency percentiles (ms) and aggregate Mrays/s."""
+    srv = ctx.server(tuning=tuning)
+    ids = {}
+    for name, (kind, d, md) in volumes.items():
+        ids[name] = (srv.load_grid(d) if kind == "grid" else srv.load_svo(*d)), kind
+    grids = [n for n, (_, k) in ids.items() if k == "grid"]
+    svos = [n for n, (_, k) in ids.items() if k == "svo"]
+    rng = np.random.default_rng(seed)
+    n = n_saturated if n_saturated else int(rate * duration_s)
+    offsets = np.zeros(n) if n_saturated else np.cumsum(rng.exponential(1000.0 / rate, n))
+    reqs = []
+    for i in range(n):
+        t = int(rng.integers(5))
+        name = (grids if t == abi.DDA else svos)[int(rng.integers(len(grids if t == abi.DDA else svos)))]
+        W = int(round(np.exp(rng.uniform(np.log(128), np.log(1920)))))
+        H = int(round(np.exp(rng.uniform(np.log(128), np.log(1080)))))
+        ang = rng.uniform(0, 2 * np.pi)
+        elev = rng.uniform(-0.6, 0.6)
+        d = rng.uniform(0.5, 3.0)
+        eye = 0.5 + d * np.array([np.cos(ang) * np.cos(elev), np.sin(elev), np.sin(ang) * np.cos(elev)])
+        reqs.append((ids[name][0], t, abi.make_camera(*look_at(eye, (0.5, 0.5, 0.5))), W, H))
+    t_base = srv.now_ms() + 20.0
+    lead_ms = 10.0
+
+    def feed():
+        for i, (vid, t, cam, W, H) in enumerate(reqs):
+            release = t_base + offsets[i]
+            wait = (release - lead_ms - srv.now_ms()) / 1000.0
+            if wait > 0:
+                time.sleep(wait)
+            srv.submit(vid, t, cam, W, H, emission=EMISSION, release_ms=release, tag=i)
+
+    th = threading.Thread(target=feed)
+    th.start()
+    done = []
+    while len(done) < n:
+        done += srv.poll(512, timeout_ms=50)
+    th.join()
+    st = srv.stats()
+    srv.free()
+    bad = [c for c in done if c["status"] != 0]
+    if bad:
+        raise RuntimeError(f"{len(bad)} served requests failed (status {bad[0]['status']})")
+    lat = np.array([c["done_ms"] - c["arrival_ms"] for c in done])
+    dev = np.array([c["device_done_ms"] - c["arrival_ms"] for c in done])
+    wall = max(c["done_ms"] for c in done) - min(c["arrival_ms"] for c in done)
+    rays = sum(c["rays"] for c in done)
+    return {"rate_req_s": rate if not n_saturated else None, "requests": n, "duration_s": round(wall / 1e3, 3),
+            "mrays_s": round(rays / wall / 1e3, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "p50_device_ms": round(float(np.percentile(dev, 50)), 3),
+            "p99_device_ms": round(float(np.percentile(dev, 99)), 3),
+            "mean_rays_per_request": round(rays / n), "launches": st["launches"],
+            "max_requests_per_launch": st["max_requests_per_launch"],
+            "device_busy_frac": round(st["busy_ms"] / wall, 3)}
+
+
 def copy_calibration(gib):
     """A plain device copy of known size, for scaling the FETCH_SIZE /
     WRITE_SIZE counters of a rocprofv3 --pmc run."""
@@ -122,6 +201,10 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a device copy of this size (HBM counter calibration)")
     ap.add_argument("--tile", type=int, default=0, help="per-wave tile width override (8/16/32)")
+    ap.add_argument("--serve-rates", default="200,1000,2000", help="Poisson request rates (req/s); empty: skip")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-saturated", type=int, default=400,
+                    help="requests released at once for the served form's peak rate (0: skip)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -268,6 +351,24 @@ def main():
                 "mean_launch_ms": round(mean_batch_ms, 4),
                 "plain_per_traversal_algorithmic_GBs": plain_roof}
 
+    # ---------------- served form: Poisson arrivals, continuous batching -----
+    served = []
+    rates = [float(r) for r in args.serve_rates.split(",") if r]
+    if rates or args.serve_saturated:
+        shells = synth.shells_128()
+        snodes, sst = abi.build_octree(shells, tree=abi.TREE_ROPE)
+        svols = {"ct512_grid": ("grid", grid, [512, 361, 512]), "ct512_svo": ("svo", (nodes, side), [side] * 3),
+                 "shells128_grid": ("grid", shells, [128] * 3),
+                 "shells128_svo": ("svo", (snodes, sst["side"]), [sst["side"]] * 3)}
+        for k, r in enumerate(rates):
+            res = served_leg(ctx, svols, r, args.serve_seconds, seed=1000 + k, tuning=tuning)
+            log(f"served @ {r:.0f} req/s: {res['mrays_s']} Mrays/s, p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms")
+            served.append(res)
+        if args.serve_saturated:
+            res = served_leg(ctx, svols, 0, 0, seed=999, tuning=tuning, n_saturated=args.serve_saturated)
+            log(f"served, {args.serve_saturated} requests at once: {res['mrays_s']} Mrays/s, p99 {res['p99_ms']} ms")
+            served.append(res)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", 0) or min(16, os.cpu_count() or 1))
@@ -295,6 +396,7 @@ def main():
                   "per_traversal_mrays": {n: round(v, 1) for n, v in per_trav.items()},
                   "per_traversal_mean_ms": {n: round(v, 4) for n, v in mean_ms.items()}},
         "batched_per_traversal_mrays": {n: round(v, 1) for n, v in per_trav_batched.items()},
+        "served": served,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

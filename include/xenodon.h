@@ -136,6 +136,12 @@ XD_API int xd_volume_from_grid(xd_context* ctx, const void* rgba8, uint64_t dim_
 XD_API int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, uint64_t side,
                               xd_volume** out);
 XD_API int xd_volume_info_get(const xd_volume* vol, xd_volume_info* out);
+/* HBM bytes a volume will hold once created (xd_volume_info.device_bytes),
+ * computed on the host without touching the device: kind XD_VOLUME_GRID uses
+ * the dimensions, XD_VOLUME_SVO the node count.  ctx may be NULL (default
+ * octree layout).  Used to admit volumes by footprint before allocating. */
+XD_API int xd_volume_footprint(const xd_context* ctx, int kind, uint64_t dim_x, uint64_t dim_y, uint64_t dim_z,
+                               uint64_t node_count, uint64_t* bytes);
 XD_API int xd_volume_free(xd_volume* vol);
 
 /* ---- frames (replaces HeadlessOutput's render target) -------------------- */
@@ -203,6 +209,89 @@ XD_API int xd_batch_autotune(xd_batch* batch, uint32_t trials, xd_batch_tuning* 
  * whole launch; *rays the number of pixels rendered. */
 XD_API int xd_batch_render(xd_batch* batch, double* ms, uint64_t* rays);
 XD_API int xd_batch_free(xd_batch* batch);
+
+/* ---- served form: request loop over resident volumes -------------------
+ *
+ * No counterpart in the reference, which renders one camera script per
+ * process.  A server owns volumes it admitted by HBM footprint and a worker
+ * thread that folds queued render requests into the running work: every
+ * scheduling step is one persistent-kernel launch over wave tiles drawn from
+ * all admitted requests (a fair share each, oldest first), with up to
+ * `launches_in_flight` launches queued on the device so that requests
+ * arriving while others run join the next launch.  Each pixel is computed by
+ * the same per-ray code as xd_render, so a result does not depend on the
+ * requests it was batched with.  Completed frames are copied to the
+ * request's host buffer by a second thread.  Times are ms on a steady clock
+ * whose zero is the server's creation. */
+
+typedef struct xd_server xd_server;
+
+typedef struct {
+    uint64_t hbm_budget_bytes;   /* resident volumes may hold at most this much;
+                                    0 = 90% of the device memory free at creation */
+    uint64_t frame_arena_bytes;  /* device memory for frames of admitted requests;
+                                    0 = 512 MiB */
+    uint32_t max_active;         /* requests admitted at once (1..1024; 0 = 256) */
+    uint32_t tiles_per_launch;   /* wave tiles per scheduling step (0 = 32768) */
+    uint32_t launches_in_flight; /* 1..4 (0 = 2) */
+    uint32_t reserved;
+    xd_batch_tuning tuning;      /* tile shapes etc.; zero-filled = defaults */
+} xd_server_config;
+
+typedef struct {
+    uint32_t volume_id;          /* from xd_server_load_grid / _svo */
+    int32_t traversal;
+    xd_camera camera;
+    uint32_t width, height;      /* the whole display; output region = display */
+    float emission_coeff;
+    float voxel_ratio[3];        /* 0 entries read as 1 */
+    uint32_t* dst;               /* host pixels, width*height RGBA8 rows; NULL = discard */
+    double release_ms;           /* arrival time on the server clock; the request
+                                    is not started before it (0 = on submit) */
+    uint64_t user_tag;
+} xd_request;
+
+typedef struct {
+    uint64_t request_id;
+    uint64_t user_tag;
+    int32_t status;              /* XD_OK, or the error that ended the request */
+    uint32_t launches;           /* scheduling steps that rendered its tiles */
+    double arrival_ms;           /* release_ms, or the submit time */
+    double start_ms;             /* first launch with its tiles was queued */
+    double device_done_ms;       /* its last tile finished on the device */
+    double done_ms;              /* frame in host memory (= device_done_ms if dst is NULL) */
+    uint64_t rays;
+} xd_completion;
+
+typedef struct {
+    uint64_t resident_bytes, budget_bytes;
+    uint32_t volumes, pending, active, reserved;
+    uint64_t completed, launches, tiles, rays;
+    uint32_t max_requests_per_launch, max_launches_overlapped;
+    double busy_ms;              /* sum of launch durations (HIP events) */
+} xd_server_stats;
+
+XD_API int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** out);
+/* Load a volume if its footprint (xd_volume_footprint) fits the remaining
+ * budget, else XD_ERR_REJECTED and nothing is allocated. */
+XD_API int xd_server_load_grid(xd_server* srv, const void* rgba8, uint64_t dim_x, uint64_t dim_y, uint64_t dim_z,
+                               uint32_t* volume_id);
+XD_API int xd_server_load_svo(xd_server* srv, const void* nodes, uint64_t node_count, uint64_t side,
+                              uint32_t* volume_id);
+/* Waits until no queued or running request uses the volume, then frees it. */
+XD_API int xd_server_unload(xd_server* srv, uint32_t volume_id);
+/* Validates and queues a request (XD_ERR_INVALID_ARGUMENT for an unknown
+ * volume or bad size, XD_ERR_INCOMPATIBLE for a traversal the volume cannot
+ * run, XD_ERR_REJECTED if its frame can never fit the arena). */
+XD_API int xd_server_submit(xd_server* srv, const xd_request* req, uint64_t* request_id);
+/* Returns up to max finished requests in *n, waiting up to timeout_ms
+ * (0 = do not wait, <0 = wait for at least one). */
+XD_API int xd_server_poll(xd_server* srv, xd_completion* out, uint32_t max, uint32_t* n, int32_t timeout_ms);
+XD_API int xd_server_stats_get(xd_server* srv, xd_server_stats* out);
+/* Current time on the server clock (ms). */
+XD_API double xd_server_now_ms(const xd_server* srv);
+/* Finishes every queued request, stops the threads, frees the volumes. */
+XD_API int xd_server_free(xd_server* srv);
 
 /* ---- octree construction (`xenodon convert`, src/convert.cpp:13-122) ----- */
 

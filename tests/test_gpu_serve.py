@@ -1,0 +1,215 @@
+"""Served form (xd_server_*): resident volumes admitted by HBM footprint,
+requests folded into the running work as they arrive, and every result
+identical to the plain render of the same view -- alone or in any mix.
+
+Results are compared bit-for-bit: each tile is shaded by the same per-ray
+code whatever it is batched with, so the stated tolerance (<= 1 LSB) is never
+needed here.
+"""
+import numpy as np
+import pytest
+
+from tests import helpers as H
+from xenodon_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+SVO_TRAVERSALS = [abi.SVO_NAIVE, abi.ESVO, abi.SVO_DF, abi.SVO_ROPE]
+
+
+@pytest.fixture(scope="module")
+def env():
+    if abi.device_count() < 1:
+        pytest.fail("no HIP device visible: GPU tests need an MI355X")
+    ctx = abi.Context(0)
+    shells = synth.shells_128()
+    nodes, st = abi.build_octree(shells, tree=abi.TREE_ROPE)
+    blocky = H.blocky_grid(64, 4, seed=41)
+    bnodes, bst = abi.build_octree(blocky, tree=abi.TREE_ROPE)
+    data = {
+        "shells_grid": ("grid", shells, [128, 128, 128]),
+        "shells_svo": ("svo", (nodes, st["side"]), [st["side"]] * 3),
+        "blocky_grid": ("grid", blocky, [64, 64, 64]),
+        "blocky_svo": ("svo", (bnodes, bst["side"]), [bst["side"]] * 3),
+    }
+    plain = {}
+    for name, (kind, d, md) in data.items():
+        plain[name] = (ctx.grid_volume(d) if kind == "grid" else ctx.svo_volume(*d)), md
+    yield ctx, data, plain
+    ctx.close()
+
+
+def _load_all(srv, data):
+    ids = {}
+    for name, (kind, d, md) in data.items():
+        ids[name] = srv.load_grid(d) if kind == "grid" else srv.load_svo(*d)
+    return ids
+
+
+def _random_requests(n, seed, data):
+    rng = np.random.default_rng(seed)
+    names = list(data)
+    cams = H.orbit_cameras(n, 1.8, seed=seed)
+    out = []
+    for i in range(n):
+        name = names[rng.integers(len(names))]
+        t = abi.DDA if data[name][0] == "grid" else SVO_TRAVERSALS[rng.integers(4)]
+        W = int(np.exp(rng.uniform(np.log(8), np.log(400))))
+        Hh = int(np.exp(rng.uniform(np.log(8), np.log(300))))
+        out.append((name, t, cams[i], W, Hh, float(rng.uniform(1.0, 4.0))))
+    return out
+
+
+def _plain(ctx, plain, name, t, cam, W, Hh, e):
+    vol, md = plain[name]
+    img, _ = ctx.render_image(vol, t, abi.make_camera(*cam), abi.make_uniforms(W, Hh, md, emission=e))
+    return img
+
+
+def _drain(srv, count):
+    got = []
+    while len(got) < count:
+        got += srv.poll(256, timeout_ms=-1)
+    return got
+
+
+def test_served_frames_match_plain(env):
+    ctx, data, plain = env
+    srv = ctx.server(tiles_per_launch=1024)
+    ids = _load_all(srv, data)
+    reqs = _random_requests(40, seed=5, data=data)
+    tags = {}
+    for k, (name, t, cam, W, Hh, e) in enumerate(reqs):
+        rid, _ = srv.submit(ids[name], t, abi.make_camera(*cam), W, Hh, emission=e, tag=k)
+        tags[rid] = k
+    done = _drain(srv, len(reqs))
+    assert sorted(c["tag"] for c in done) == list(range(len(reqs)))
+    for c in done:
+        assert c["status"] == 0
+        name, t, cam, W, Hh, e = reqs[c["tag"]]
+        assert c["rays"] == W * Hh
+        assert c["arrival_ms"] <= c["start_ms"] <= c["device_done_ms"] <= c["done_ms"]
+        assert np.array_equal(c["image"], _plain(ctx, plain, name, t, cam, W, Hh, e)), (name, t, W, Hh)
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["rays"] == sum(r[3] * r[4] for r in reqs)
+    assert st["max_requests_per_launch"] >= 2  # requests shared scheduling steps
+    srv.free()
+
+
+def test_result_is_the_same_alone_and_in_the_mix(env):
+    ctx, data, plain = env
+    probe = ("shells_svo", abi.SVO_DF, H.orbit_cameras(1, 1.7, seed=99)[0], 333, 217, 3.0)
+    name, t, cam, W, Hh, e = probe
+    # alone
+    srv = ctx.server(tiles_per_launch=256)
+    ids = _load_all(srv, data)
+    srv.submit(ids[name], t, abi.make_camera(*cam), W, Hh, emission=e, tag=1)
+    alone = _drain(srv, 1)[0]
+    assert alone["status"] == 0
+    # the same request arriving in the middle of 60 others
+    mix = _random_requests(60, seed=11, data=data)
+    for k, (n2, t2, c2, W2, H2, e2) in enumerate(mix):
+        srv.submit(ids[n2], t2, abi.make_camera(*c2), W2, H2, emission=e2, tag=100 + k)
+        if k == 30:
+            srv.submit(ids[name], t, abi.make_camera(*cam), W, Hh, emission=e, tag=2)
+    done = {c["tag"]: c for c in _drain(srv, len(mix) + 1)}
+    mixed = done[2]
+    assert mixed["status"] == 0
+    assert mixed["launches"] > 1  # spread over several steps shared with other requests
+    assert np.array_equal(alone["image"], mixed["image"])
+    assert np.array_equal(alone["image"], _plain(ctx, plain, name, t, cam, W, Hh, e))
+    srv.free()
+
+
+def test_late_small_request_overtakes_a_large_one(env):
+    """Continuous batching, not a FIFO queue: a small request arriving while a
+    1080p frame is on the GPU is folded into the next steps and finishes first."""
+    ctx, data, plain = env
+    srv = ctx.server(tiles_per_launch=512)
+    ids = _load_all(srv, data)
+    cam = abi.make_camera(*H.orbit_cameras(1, 1.5, seed=3)[0])
+    big, _ = srv.submit(ids["shells_svo"], abi.SVO_DF, cam, 1920, 1080, emission=2.0, tag=1)
+    small, _ = srv.submit(ids["blocky_grid"], abi.DDA, cam, 64, 64, emission=2.0, tag=2)
+    done = {c["tag"]: c for c in _drain(srv, 2)}
+    assert done[2]["done_ms"] < done[1]["device_done_ms"]
+    assert done[1]["launches"] > 10
+    st = srv.stats()
+    assert st["max_requests_per_launch"] >= 2
+    srv.free()
+
+
+def test_admission_by_hbm_footprint(env):
+    ctx, data, plain = env
+    shells = data["shells_grid"][1]
+    fp = abi.volume_footprint(abi.VOLUME_GRID, (128, 128, 128))
+    assert fp == plain["shells_grid"][0].info().device_bytes
+    nodes, side = data["shells_svo"][1]
+    assert abi.volume_footprint(abi.VOLUME_SVO, node_count=nodes.shape[0]) == \
+        plain["shells_svo"][0].info().device_bytes
+    srv = ctx.server(hbm_budget_bytes=fp + fp // 2)
+    a = srv.load_grid(shells)
+    assert srv.stats()["resident_bytes"] == fp
+    with pytest.raises(abi.XdError) as e:
+        srv.load_grid(shells)
+    assert e.value.status == abi.XD_ERR_REJECTED
+    assert srv.stats()["resident_bytes"] == fp and srv.stats()["volumes"] == 1
+    srv.unload(a)
+    assert srv.stats()["resident_bytes"] == 0
+    b = srv.load_grid(shells)
+    rid, img = srv.submit(b, abi.DDA, abi.make_camera(*H.orbit_cameras(1, 2.0, seed=4)[0]), 64, 48, emission=3.0)
+    c = _drain(srv, 1)[0]
+    assert c["status"] == 0 and c["id"] == rid
+    srv.free()
+
+
+def test_bad_requests_are_refused_at_submit(env):
+    ctx, data, plain = env
+    srv = ctx.server(frame_arena_bytes=1 << 20)
+    ids = _load_all(srv, data)
+    cam = abi.make_camera((0, 0, 1), (0, 1, 0), (0.5, 0.5, -2))
+    cases = [
+        (dict(volume_id=999, traversal=abi.DDA, width=8, height=8), abi.XD_ERR_INVALID_ARGUMENT),
+        (dict(volume_id=ids["shells_svo"], traversal=abi.DDA, width=8, height=8), abi.XD_ERR_INCOMPATIBLE),
+        (dict(volume_id=ids["shells_grid"], traversal=abi.ESVO, width=8, height=8), abi.XD_ERR_INCOMPATIBLE),
+        (dict(volume_id=ids["shells_grid"], traversal=7, width=8, height=8), abi.XD_ERR_INVALID_ARGUMENT),
+        (dict(volume_id=ids["shells_grid"], traversal=abi.DDA, width=0, height=8), abi.XD_ERR_INVALID_ARGUMENT),
+        (dict(volume_id=ids["shells_grid"], traversal=abi.DDA, width=1920, height=1080), abi.XD_ERR_REJECTED),
+    ]
+    for kw, status in cases:
+        with pytest.raises(abi.XdError) as e:
+            srv.submit(kw["volume_id"], kw["traversal"], cam, kw["width"], kw["height"])
+        assert e.value.status == status, (kw, e.value)
+    with pytest.raises(abi.XdError):
+        srv.poll(4, timeout_ms=-1)  # nothing queued: refuses to wait forever
+    assert srv.poll(4, timeout_ms=0) == []
+    # the server still works after refusals
+    srv.submit(ids["blocky_grid"], abi.DDA, cam, 32, 32)
+    assert _drain(srv, 1)[0]["status"] == 0
+    srv.free()
+
+
+def test_release_times_are_honoured(env):
+    ctx, data, plain = env
+    srv = ctx.server()
+    ids = _load_all(srv, data)
+    cam = abi.make_camera(*H.orbit_cameras(1, 2.0, seed=8)[0])
+    t0 = srv.now_ms()
+    for k in range(5):
+        srv.submit(ids["blocky_svo"], abi.ESVO, cam, 96, 64, release_ms=t0 + 30.0 + 10.0 * k, tag=k)
+    done = sorted(_drain(srv, 5), key=lambda c: c["tag"])
+    for k, c in enumerate(done):
+        assert c["arrival_ms"] == pytest.approx(t0 + 30.0 + 10.0 * k)
+        assert c["start_ms"] >= c["arrival_ms"]
+    srv.free()
+
+
+def test_free_finishes_queued_requests(env):
+    ctx, data, plain = env
+    srv = ctx.server()
+    ids = _load_all(srv, data)
+    cam = abi.make_camera(*H.orbit_cameras(1, 2.0, seed=9)[0])
+    keep = [srv.submit(ids["shells_grid"], abi.DDA, cam, 200, 150, emission=2.0)[1] for _ in range(4)]
+    srv.free()  # drains the queue before stopping
+    ref = _plain(ctx, plain, "shells_grid", abi.DDA, H.orbit_cameras(1, 2.0, seed=9)[0], 200, 150, 2.0)
+    for img in keep:
+        assert np.array_equal(img, ref)

@@ -14,6 +14,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libxenodon_amd.so")
 
 XD_OK = 0
+(XD_ERR_INVALID_ARGUMENT, XD_ERR_DEVICE, XD_ERR_OUT_OF_MEMORY, XD_ERR_INCOMPATIBLE, XD_ERR_FORMAT, XD_ERR_IO,
+ XD_ERR_REJECTED, XD_ERR_INTERNAL) = range(1, 9)
 DDA, SVO_NAIVE, ESVO, SVO_DF, SVO_ROPE = range(5)
 # src/main_loop.cpp:37-43 option names
 TRAVERSALS = {"dda": DDA, "svo-naive": SVO_NAIVE, "esvo": ESVO, "svo-df": SVO_DF, "svo-rope": SVO_ROPE}
@@ -75,6 +77,33 @@ class BuildStats(ctypes.Structure):
                 ("total_leaves", "unique_leaves", "total_nodes", "depth", "side", "node_count")]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("frame_arena_bytes", ctypes.c_uint64),
+                ("max_active", ctypes.c_uint32), ("tiles_per_launch", ctypes.c_uint32),
+                ("launches_in_flight", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("tuning", BatchTuning)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("volume_id", ctypes.c_uint32), ("traversal", ctypes.c_int32), ("camera", Camera),
+                ("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("emission_coeff", ctypes.c_float),
+                ("voxel_ratio", ctypes.c_float * 3), ("dst", ctypes.c_void_p), ("release_ms", ctypes.c_double),
+                ("user_tag", ctypes.c_uint64)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("request_id", ctypes.c_uint64), ("user_tag", ctypes.c_uint64), ("status", ctypes.c_int32),
+                ("launches", ctypes.c_uint32), ("arrival_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("device_done_ms", ctypes.c_double), ("done_ms", ctypes.c_double), ("rays", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("resident_bytes", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
+                ("volumes", ctypes.c_uint32), ("pending", ctypes.c_uint32), ("active", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("completed", ctypes.c_uint64), ("launches", ctypes.c_uint64),
+                ("tiles", ctypes.c_uint64), ("rays", ctypes.c_uint64), ("max_requests_per_launch", ctypes.c_uint32),
+                ("max_launches_overlapped", ctypes.c_uint32), ("busy_ms", ctypes.c_double)]
+
+
 HEURISTIC_CHAN_DIFF, HEURISTIC_STD_DEV = 0, 1
 TREE_SPARSE, TREE_DAG, TREE_ROPE = 0, 1, 2
 
@@ -113,6 +142,18 @@ def _declare(L):
         "xd_batch_autotune": ([vp, ctypes.c_uint32, P(BatchTuning)], ctypes.c_int),
         "xd_batch_render": ([vp, P(ctypes.c_double), P(ctypes.c_uint64)], ctypes.c_int),
         "xd_batch_free": ([vp], ctypes.c_int),
+        "xd_volume_footprint": ([vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                 ctypes.c_uint64, P(ctypes.c_uint64)], ctypes.c_int),
+        "xd_server_create": ([vp, P(ServerConfig), P(vp)], ctypes.c_int),
+        "xd_server_load_grid": ([vp, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_uint32)],
+                                ctypes.c_int),
+        "xd_server_load_svo": ([vp, vp, ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_uint32)], ctypes.c_int),
+        "xd_server_unload": ([vp, ctypes.c_uint32], ctypes.c_int),
+        "xd_server_submit": ([vp, P(Request), P(ctypes.c_uint64)], ctypes.c_int),
+        "xd_server_poll": ([vp, P(Completion), ctypes.c_uint32, P(ctypes.c_uint32), ctypes.c_int32], ctypes.c_int),
+        "xd_server_stats_get": ([vp, P(ServerStats)], ctypes.c_int),
+        "xd_server_now_ms": ([vp], ctypes.c_double),
+        "xd_server_free": ([vp], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -220,6 +261,22 @@ class Context:
         check(lib().xd_batch_create(self.h, ctypes.byref(h)))
         return self._adopt(Batch(self, h))
 
+    def server(self, hbm_budget_bytes=0, frame_arena_bytes=0, max_active=0, tiles_per_launch=0,
+               launches_in_flight=0, tuning=None):
+        c = ServerConfig()
+        c.hbm_budget_bytes = hbm_budget_bytes
+        c.frame_arena_bytes = frame_arena_bytes
+        c.max_active = max_active
+        c.tiles_per_launch = tiles_per_launch
+        c.launches_in_flight = launches_in_flight
+        if tuning:
+            c.tuning.tile_w[:] = list(tuning.get("tile_w", [8] * 5))
+            c.tuning.tiles_per_grab = tuning.get("tiles_per_grab", 1)
+            c.tuning.waves_per_simd = tuning.get("waves_per_simd", 0)
+        h = ctypes.c_void_p()
+        check(lib().xd_server_create(self.h, ctypes.byref(c), ctypes.byref(h)))
+        return self._adopt(Server(self, h))
+
     def set_tile(self, tile_w):
         check(lib().xd_context_set_tile(self.h, tile_w))
 
@@ -308,6 +365,93 @@ class Batch:
             self.free()
         except Exception:
             pass
+
+
+class Server:
+    """Served form (xd_server_*): resident volumes admitted by HBM footprint
+    and a request loop with continuous batching."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+        self._dst = {}  # request id -> host array the server writes into
+
+    def load_grid(self, grid):
+        g = np.ascontiguousarray(grid, dtype=np.uint8)
+        vid = ctypes.c_uint32()
+        check(lib().xd_server_load_grid(self.h, g.ctypes.data, g.shape[2], g.shape[1], g.shape[0],
+                                        ctypes.byref(vid)))
+        return vid.value
+
+    def load_svo(self, nodes, side):
+        n = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 10)
+        vid = ctypes.c_uint32()
+        check(lib().xd_server_load_svo(self.h, n.ctypes.data, n.shape[0], int(side), ctypes.byref(vid)))
+        return vid.value
+
+    def unload(self, volume_id):
+        check(lib().xd_server_unload(self.h, volume_id))
+
+    def submit(self, volume_id, traversal, cam, width, height, emission=1.0, voxel_ratio=(1.0, 1.0, 1.0),
+               keep=True, release_ms=0.0, tag=0):
+        """Queues a request; returns (request id, host array it fills, or None
+        when keep=False)."""
+        r = Request()
+        r.volume_id = volume_id
+        r.traversal = traversal
+        r.camera = cam
+        r.width, r.height = width, height
+        r.emission_coeff = emission
+        r.voxel_ratio[:] = list(voxel_ratio)
+        dst = np.zeros((height, width, 4), np.uint8) if keep else None
+        r.dst = dst.ctypes.data if keep else None
+        r.release_ms = release_ms
+        r.user_tag = tag
+        rid = ctypes.c_uint64()
+        check(lib().xd_server_submit(self.h, ctypes.byref(r), ctypes.byref(rid)))
+        if keep:
+            self._dst[rid.value] = dst
+        return rid.value, dst
+
+    def poll(self, max_n=256, timeout_ms=-1):
+        """Finished requests as dicts (status, times in ms, rays, image)."""
+        arr = (Completion * max_n)()
+        n = ctypes.c_uint32()
+        check(lib().xd_server_poll(self.h, arr, max_n, ctypes.byref(n), timeout_ms))
+        out = []
+        for i in range(n.value):
+            c = arr[i]
+            out.append({"id": c.request_id, "tag": c.user_tag, "status": c.status, "launches": c.launches,
+                        "arrival_ms": c.arrival_ms, "start_ms": c.start_ms, "device_done_ms": c.device_done_ms,
+                        "done_ms": c.done_ms, "rays": c.rays, "image": self._dst.pop(c.request_id, None)})
+        return out
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().xd_server_stats_get(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_ if k != "reserved"}
+
+    def now_ms(self):
+        return lib().xd_server_now_ms(self.h)
+
+    def free(self):
+        if self.h:
+            lib().xd_server_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def volume_footprint(kind, dims=(0, 0, 0), node_count=0, ctx=None):
+    """HBM bytes a volume will hold (xd_volume_footprint); needs no device."""
+    b = ctypes.c_uint64()
+    check(lib().xd_volume_footprint(ctx.h if ctx else None, kind, dims[0], dims[1], dims[2], node_count,
+                                    ctypes.byref(b)))
+    return b.value
 
 
 class Volume:

@@ -20,22 +20,10 @@
 #include <cstring>
 #include <vector>
 
-#include "xd_runtime.hpp"
+#include "xd_batch.hpp"
 #include "xd_trace.hpp"
 
 namespace xd {
-
-struct BatchView {
-    RenderParams p;
-    uint32_t traversal;
-    uint32_t tw_shift;  // log2 of the tile width
-    uint32_t tiles_x;
-    uint32_t pad;
-};
-
-constexpr uint32_t kQueues = 8;
-constexpr uint32_t kMaxBatchViews = 1u << 12;
-constexpr uint32_t kMaxTilesPerView = 1u << 20;
 
 __global__ void k_batch_reset(uint32_t* counters) {
     if (threadIdx.x < kQueues) counters[threadIdx.x] = 0;
@@ -123,6 +111,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     batch_body(views, tiles, qbeg, counters, per_grab, lds);
 }
 
+uint32_t tw_shift_of(uint32_t tw) { return tw == 32 ? 5u : (tw == 16 ? 4u : 3u); }
+
+void make_batch_view(const xd_volume* vol, int traversal, const xd_camera* cam, const xd_uniforms* u, uint32_t* out,
+                     uint32_t pitch, uint32_t tile_w, BatchView& B, uint64_t* ntiles) {
+    memset(&B, 0, sizeof(B));
+    fill_params(B.p, vol, traversal, cam, u);
+    B.p.top_nodes = 0;  // no LDS top cache: views of one launch may use different volumes
+    B.p.out = out;
+    B.p.out_pitch = pitch;
+    B.traversal = (uint32_t)traversal;
+    if (tile_w != 8 && tile_w != 16 && tile_w != 32) tile_w = 8;
+    B.tw_shift = tw_shift_of(tile_w);
+    const uint32_t th = 64u / tile_w;
+    B.tiles_x = (u->output_region.extent[0] + tile_w - 1) / tile_w;
+    *ntiles = (uint64_t)B.tiles_x * ((u->output_region.extent[1] + th - 1) / th);
+}
+
+uint32_t batch_view_slots(const BatchView& B) {
+    return (B.traversal == XD_ESVO || B.traversal == XD_SVO_DF) ? B.p.stack_slots : 0u;
+}
+
+size_t batch_lds_bytes(uint32_t slots) {
+    return (size_t)2 * std::max<uint32_t>(slots, 1) * kBlockThreads * sizeof(uint32_t);  // 4 waves x own stacks
+}
+
+int batch_grid(int device, uint32_t slots, bool w8, uint32_t* grid) {
+    hipError_t e;
+    int cus = 0, per_cu = 0;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess ||
+        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+             &per_cu, reinterpret_cast<const void*>(w8 ? k_batch_w8 : k_batch), 256, batch_lds_bytes(slots))) !=
+            hipSuccess)
+        return fail_hip(e, "batch occupancy");
+    uint32_t g = (uint32_t)std::max(1, cus * std::max(per_cu, 1));
+    *grid = (g + kQueues - 1) / kQueues * kQueues;
+    return XD_OK;
+}
+
+hipError_t launch_batch(const BatchView* views, const uint32_t* tiles, const uint32_t* qbeg, uint32_t* counters,
+                        uint32_t grab, uint32_t slots, bool w8, uint32_t grid, hipStream_t s) {
+    const size_t lds = batch_lds_bytes(slots);
+    grab = std::max<uint32_t>(1, std::min<uint32_t>(grab, 8));
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counters);
+    if (w8)
+        hipLaunchKernelGGL(k_batch_w8, dim3(grid), dim3(256), lds, s, views, tiles, qbeg, counters, grab);
+    else
+        hipLaunchKernelGGL(k_batch, dim3(grid), dim3(256), lds, s, views, tiles, qbeg, counters, grab);
+    return hipGetLastError();
+}
+
 }  // namespace xd
 
 using namespace xd;
@@ -146,8 +184,6 @@ struct xd_batch {
 };
 
 namespace {
-
-uint32_t tw_shift_of(uint32_t tw) { return tw == 32 ? 5u : (tw == 16 ? 4u : 3u); }
 
 void drop_graph(xd_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -187,24 +223,15 @@ int upload_tables(xd_batch* b, uint32_t* max_slots) {
     for (uint32_t i = 0; i < n; ++i) {
         const xd_view& v = b->views[i];
         BatchView& B = bv[i];
-        memset(&B, 0, sizeof(B));
-        fill_params(B.p, v.volume, v.traversal, &v.camera, &v.uniforms);
-        B.p.top_nodes = 0;  // no LDS top cache: views may use different volumes
-        B.p.out = v.frame->d_pixels;
-        B.p.out_pitch = v.frame->w;
-        B.traversal = (uint32_t)v.traversal;
-        uint32_t tw = b->tuning.tile_w[v.traversal];
-        if (tw != 8 && tw != 16 && tw != 32) tw = 8;
-        B.tw_shift = tw_shift_of(tw);
-        const uint32_t th = 64u / tw;
+        uint64_t nt = 0;
+        make_batch_view(v.volume, v.traversal, &v.camera, &v.uniforms, v.frame->d_pixels, v.frame->w,
+                        b->tuning.tile_w[v.traversal], B, &nt);
         const uint32_t W = v.uniforms.output_region.extent[0], H = v.uniforms.output_region.extent[1];
-        B.tiles_x = (W + tw - 1) / tw;
-        const uint32_t tiles_y = (H + th - 1) / th;
-        const uint64_t nt = (uint64_t)B.tiles_x * tiles_y;
+        const uint32_t tiles_y = (uint32_t)(nt / B.tiles_x);
         if (nt > kMaxTilesPerView)
             return fail(XD_ERR_INVALID_ARGUMENT, "view %u has %llu tiles (limit %u)", i, (unsigned long long)nt,
                         kMaxTilesPerView);
-        if (v.traversal == XD_ESVO || v.traversal == XD_SVO_DF) slots = std::max(slots, B.p.stack_slots);
+        slots = std::max(slots, batch_view_slots(B));
         for (uint32_t ty = 0; ty < tiles_y; ++ty) {
             const uint32_t band = (uint32_t)((uint64_t)ty * kQueues / tiles_y);
             for (uint32_t tx = 0; tx < B.tiles_x; ++tx) q[band].push_back((i << 20) | (ty * B.tiles_x + tx));
@@ -238,27 +265,14 @@ int upload_tables(xd_batch* b, uint32_t* max_slots) {
 int capture(xd_batch* b, uint32_t slots) {
     drop_graph(b);
     hipError_t e;
-    const size_t lds = (size_t)2 * slots * kBlockThreads * sizeof(uint32_t);  // 4 waves x own stacks
-    int dev = b->ctx->device, cus = 0, per_cu = 0;
-    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess ||
-        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-             &per_cu, reinterpret_cast<const void*>(b->tuning.waves_per_simd == 8 ? k_batch_w8 : k_batch), 256,
-             lds)) != hipSuccess)
-        return fail_hip(e, "batch occupancy");
-    b->grid = (uint32_t)std::max(1, cus * std::max(per_cu, 1));
-    b->grid = (b->grid + kQueues - 1) / kQueues * kQueues;
+    const bool w8 = b->tuning.waves_per_simd == 8;
+    int rc = batch_grid(b->ctx->device, slots, w8, &b->grid);
+    if (rc) return rc;
     hipStream_t s = b->ctx->stream;
     const uint32_t grab = std::max<uint32_t>(1, std::min<uint32_t>(b->tuning.tiles_per_grab, 8));
     if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return fail_hip(e, "hipStreamBeginCapture");
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->d_counters);
-    if (b->tuning.waves_per_simd == 8)
-        hipLaunchKernelGGL(k_batch_w8, dim3(b->grid), dim3(256), lds, s, b->d_views, b->d_tiles, b->d_qbeg,
-                           b->d_counters, grab);
-    else
-        hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(256), lds, s, b->d_views, b->d_tiles, b->d_qbeg,
-                           b->d_counters, grab);
-    hipError_t le = hipGetLastError();
+    const hipError_t le = launch_batch(b->d_views, b->d_tiles, b->d_qbeg, b->d_counters, grab, slots, w8, b->grid, s);
     if ((e = hipStreamEndCapture(s, &b->graph)) != hipSuccess) return fail_hip(e, "hipStreamEndCapture");
     if (le != hipSuccess) return fail_hip(le, "batch launch capture");
     if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess)

@@ -1,0 +1,49 @@
+// Persistent ray-tile-queue kernel shared by the batched form (xd_batch.hip)
+// and the served form (xd_serve.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "xd_runtime.hpp"
+
+namespace xd {
+
+// One output of a launch: its kernel parameters and wave-tile geometry.
+struct BatchView {
+    RenderParams p;
+    uint32_t traversal;
+    uint32_t tw_shift;  // log2 of the tile width
+    uint32_t tiles_x;
+    uint32_t pad;
+};
+
+constexpr uint32_t kQueues = 8;                 // one tile range per XCD
+constexpr uint32_t kMaxBatchViews = 1u << 12;   // a tile entry is (view << 20 | tile)
+constexpr uint32_t kMaxTilesPerView = 1u << 20;
+
+uint32_t tw_shift_of(uint32_t tile_w);
+
+// Fills B for one output (frame pixels at out with row pitch `pitch`) and
+// returns its number of wave tiles in *ntiles.  The LDS stack stride
+// B.p.lds_slots is left at the view's own depth; the launch sets it.
+void make_batch_view(const xd_volume* vol, int traversal, const xd_camera* cam, const xd_uniforms* u, uint32_t* out,
+                     uint32_t pitch, uint32_t tile_w, BatchView& B, uint64_t* ntiles);
+
+// Stack slots a view needs in LDS (0 for traversals without a stack).
+uint32_t batch_view_slots(const BatchView& B);
+
+// Dynamic LDS bytes of a launch whose deepest stack has `slots` entries.
+size_t batch_lds_bytes(uint32_t slots);
+
+// Resident blocks of the persistent kernel on the whole device (a multiple
+// of kQueues) for this LDS size and register budget.
+int batch_grid(int device, uint32_t slots, bool w8, uint32_t* grid);
+
+// Enqueues the counter reset and the persistent kernel on s.  views[i].p.lds_slots
+// must equal `slots` for every view; tiles holds qbeg[kQueues] entries split
+// into kQueues ranges.
+hipError_t launch_batch(const BatchView* views, const uint32_t* tiles, const uint32_t* qbeg, uint32_t* counters,
+                        uint32_t grab, uint32_t slots, bool w8, uint32_t grid, hipStream_t s);
+
+}  // namespace xd

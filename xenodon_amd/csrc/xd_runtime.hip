@@ -112,6 +112,36 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
     }
 }
 
+uint32_t macro_geometry(const uint64_t dims[3], uint32_t* shift, uint32_t mn[3]) {
+    // macro cells: the smallest power-of-two side >= 8 voxels whose bitmask
+    // fits the kernel's LDS staging area
+    for (uint32_t s = 3;; ++s) {
+        uint64_t cells = 1;
+        for (int i = 0; i < 3; ++i) {
+            mn[i] = (uint32_t)((dims[i] + (1ull << s) - 1) >> s);
+            cells *= mn[i];
+        }
+        const uint64_t words = (cells + 31) / 32;
+        if (words <= kMaxMacroWords) {
+            *shift = s;
+            return (uint32_t)words;
+        }
+    }
+}
+
+uint64_t grid_footprint(const uint64_t dims[3]) {
+    uint64_t nbricks = 1;
+    for (int i = 0; i < 3; ++i) nbricks *= (dims[i] + kBrick - 1) / kBrick;
+    uint32_t shift, mn[3];
+    const uint32_t words = macro_geometry(dims, &shift, mn);
+    return nbricks * 64 * sizeof(uint32_t) + nbricks + (uint64_t)words * 4;
+}
+
+uint64_t svo_footprint(uint64_t node_count, bool raw) {
+    if (raw || node_count > kMaxTaggedNodes) return node_count * kNodeWords * sizeof(uint32_t);
+    return node_count * 8 * sizeof(uint32_t) + node_count * sizeof(uint2);
+}
+
 int check_view(const xd_uniforms* u) {
     if (u->output_region.extent[0] == 0 || u->output_region.extent[1] == 0)
         return fail(XD_ERR_INVALID_ARGUMENT, "output region is empty");
@@ -203,7 +233,9 @@ int xd_close(xd_context* ctx) {
     if (!ctx) return XD_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    // batches, volumes and frames still alive are released with their context
+    // servers, batches, volumes and frames still alive are released with their context
+    std::vector<xd_server*> srvs(ctx->servers.begin(), ctx->servers.end());
+    for (auto* s : srvs) (void)xd_server_free(s);
     std::vector<xd_batch*> bats(ctx->batches.begin(), ctx->batches.end());
     for (auto* b : bats) xd_batch_free(b);
     std::vector<xd_volume*> vols(ctx->volumes.begin(), ctx->volumes.end());
@@ -262,17 +294,7 @@ int xd_volume_from_grid(xd_context* ctx, const void* rgba8, uint64_t dim_x, uint
         }
     }
     (void)hipFree(staging);
-    // macro cells: the smallest power-of-two side >= 8 voxels whose bitmask
-    // fits the kernel's LDS staging area
-    for (v->macro_shift = 3;; ++v->macro_shift) {
-        uint64_t cells = 1;
-        for (int i = 0; i < 3; ++i) {
-            v->mn[i] = (uint32_t)((v->dims[i] + (1ull << v->macro_shift) - 1) >> v->macro_shift);
-            cells *= v->mn[i];
-        }
-        v->macro_words = (uint32_t)((cells + 31) / 32);
-        if (v->macro_words <= kMaxMacroWords) break;
-    }
+    v->macro_words = macro_geometry(v->dims, &v->macro_shift, v->mn);
     if ((e = hipMalloc(&v->d_macro, (size_t)v->macro_words * 4)) != hipSuccess ||
         (e = launch_macro_mask(v->d_occ, v->nb, v->macro_shift, v->mn, v->d_macro, v->macro_words, ctx->stream)) !=
             hipSuccess ||
@@ -401,6 +423,26 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
     ctx->volumes.insert(v);
     clear_error();
     *out = v;
+    return XD_OK;
+}
+
+int xd_volume_footprint(const xd_context* ctx, int kind, uint64_t dim_x, uint64_t dim_y, uint64_t dim_z,
+                        uint64_t node_count, uint64_t* bytes) {
+    if (!bytes) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    *bytes = 0;
+    if (kind == XD_VOLUME_GRID) {
+        if (dim_x == 0 || dim_y == 0 || dim_z == 0) return fail(XD_ERR_INVALID_ARGUMENT, "grid has a zero dimension");
+        if (dim_x > (1u << 30) || dim_y > (1u << 30) || dim_z > (1u << 30))
+            return fail(XD_ERR_INVALID_ARGUMENT, "grid dimension above 2^30");
+        const uint64_t dims[3] = {dim_x, dim_y, dim_z};
+        *bytes = grid_footprint(dims);
+    } else if (kind == XD_VOLUME_SVO) {
+        if (node_count == 0 || node_count > 0xFFFFFFFFull)
+            return fail(XD_ERR_FORMAT, "octree node count %llu is not in 1..2^32-1", (unsigned long long)node_count);
+        *bytes = svo_footprint(node_count, ctx && ctx->raw_octrees);
+    } else {
+        return fail(XD_ERR_INVALID_ARGUMENT, "unknown volume kind %d", kind);
+    }
     return XD_OK;
 }
 

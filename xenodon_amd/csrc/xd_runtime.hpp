@@ -20,6 +20,7 @@ struct xd_context {
     std::unordered_set<struct xd_volume*> volumes;
     std::unordered_set<struct xd_frame*> frames;
     std::unordered_set<struct xd_batch*> batches;
+    std::unordered_set<struct xd_server*> servers;
 };
 
 struct xd_volume {
@@ -62,6 +63,11 @@ void clear_error();
 const char* traversal_name(int t);
 int check_traversal(int traversal, const xd_volume* vol);
 int check_view(const xd_uniforms* u);
+// Macro-cell side (log2) and per-axis counts of a grid; returns bitmask words.
+uint32_t macro_geometry(const uint64_t dims[3], uint32_t* shift, uint32_t mn[3]);
+// HBM bytes xd_volume_from_grid / xd_volume_from_svo keep allocated.
+uint64_t grid_footprint(const uint64_t dims[3]);
+uint64_t svo_footprint(uint64_t node_count, bool raw);
 uint64_t iteration_cap(int traversal, const xd_volume* vol, const uint32_t model_dim[3]);
 void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_camera* cam, const xd_uniforms* u);
 // Host-side derivation of the tagged, sibling-grouped device layout

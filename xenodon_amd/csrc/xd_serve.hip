@@ -1,0 +1,645 @@
+// Served form: a request loop over resident volumes with continuous batching.
+//
+// Threads:
+//   caller(s)  xd_server_submit validates a request, builds its kernel
+//              parameters and queues it; xd_server_poll collects results.
+//   worker     admits queued requests (a slot and room for the frame in the
+//              device frame arena), then repeatedly builds one scheduling
+//              step: wave tiles from every admitted request -- a fair share
+//              each, oldest request first -- become one launch of the
+//              persistent ray-tile kernel.  Up to `launches_in_flight` steps
+//              are queued on the device (each on its own stream, with its
+//              own view/tile tables), so requests that arrive while others
+//              run are folded into the next step and their tiles overlap on
+//              the device with the tail of the running one.
+//   reader     copies finished frames to the callers' host buffers and
+//              releases the request's slot and arena space.
+//
+// A tile is shaded by the same per-ray code as the plain and batched forms,
+// so a result is bit-identical however it was batched.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "xd_batch.hpp"
+
+using namespace xd;
+
+namespace {
+
+constexpr uint32_t kMaxActive = 1024;
+constexpr uint32_t kMaxRequestSide = 16384;
+constexpr uint32_t kMaxRing = 4;
+constexpr uint64_t kArenaAlign = 256;
+
+struct ServedVolume {
+    xd_volume* v = nullptr;
+    uint64_t bytes = 0;
+    uint32_t refs = 0;  // queued + admitted requests using it
+};
+
+struct Request {
+    uint64_t id = 0;
+    xd_request r{};
+    uint32_t vol_id = 0;
+    uint64_t tiles_total = 0, tiles_sched = 0, tiles_done = 0;
+    uint64_t frame_bytes = 0, arena_off = 0;
+    BatchView bv{};
+    xd_completion c{};
+};
+
+struct Part {
+    Request* req;
+    uint32_t first, count;
+};
+
+// One scheduling step in flight: its stream, events and tables.
+struct Step {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint8_t* host = nullptr;  // pinned staging of the tables
+    uint8_t* dev = nullptr;
+    std::vector<Part> parts;
+    bool busy = false;
+    uint64_t seq = 0;
+};
+
+// First-fit allocator over the frame arena (byte offsets).
+class Arena {
+    std::map<uint64_t, uint64_t> free_;  // offset -> length
+public:
+    void reset(uint64_t size) {
+        free_.clear();
+        free_[0] = size;
+    }
+    bool alloc(uint64_t n, uint64_t* off) {
+        n = (n + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            *off = it->first;
+            const uint64_t rest = it->second - n, at = it->first + n;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t n) {
+        n = (n + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct xd_server {
+    xd_context* ctx = nullptr;
+    xd_server_config cfg{};
+    std::chrono::steady_clock::time_point t0;
+
+    std::mutex mu;
+    std::condition_variable cv_work;  // worker: new request, freed slot, stop
+    std::condition_variable cv_read;  // reader: finished request, stop
+    std::condition_variable cv_done;  // callers: completion, unload progress
+
+    // volumes
+    std::map<uint32_t, ServedVolume> vols;
+    uint32_t next_vol = 1;
+    uint64_t resident = 0;  // admitted footprints (including loads in progress)
+
+    // requests
+    uint64_t next_id = 1;
+    // submitted, not admitted, ordered by arrival time (then submission)
+    std::multimap<double, Request*> pending;
+    std::vector<Request*> active;   // admitted (own arena space), not completed
+    std::deque<Request*> readq;     // all tiles done, awaiting readback
+    std::deque<xd_completion> done;
+
+    // device resources
+    uint8_t* arena_dev = nullptr;
+    Arena arena;
+    hipStream_t read_stream = nullptr;
+    Step steps[kMaxRing];
+    uint32_t ring = 2;
+    uint64_t step_seq = 0;
+    size_t views_off = 0, qbeg_off = 0, counters_off = 0, tiles_off = 0, table_bytes = 0;
+    std::map<uint32_t, uint32_t> grid_of_slots;
+    uint32_t tile_w[5] = {8, 8, 8, 8, 8};
+    uint32_t grab = 1;
+    bool w8 = false;
+
+    // state
+    bool stop = false, reader_stop = false;
+    std::string broken;  // first device error; the server then refuses work
+    xd_server_stats st{};
+
+    std::thread worker, reader;
+};
+
+namespace {
+
+// Ends every queued or admitted request with `status` (device failure),
+// except those whose tiles are all done: the reader owns and finishes them.
+void abort_all(xd_server* s, int status) {
+    auto finish = [&](Request* q) {
+        q->c.status = status;
+        q->c.done_ms = ms_since(s->t0);
+        s->vols[q->vol_id].refs--;
+        s->done.push_back(q->c);
+        delete q;
+    };
+    for (auto& kv : s->pending) finish(kv.second);
+    s->pending.clear();
+    std::vector<Request*> keep;
+    for (Request* q : s->active) {
+        if (q->tiles_done == q->tiles_total) {
+            keep.push_back(q);
+        } else {
+            s->arena.release(q->arena_off, q->frame_bytes);
+            finish(q);
+        }
+    }
+    s->active.swap(keep);
+    s->cv_done.notify_all();
+}
+
+void admit(xd_server* s, double now) {
+    for (auto it = s->pending.begin(); it != s->pending.end();) {
+        Request* q = it->second;
+        if (q->c.arrival_ms > now) break;  // not released yet (and neither is any later one)
+        if (s->active.size() >= s->cfg.max_active) break;
+        uint64_t off = 0;
+        if (!s->arena.alloc(q->frame_bytes, &off)) break;  // oldest first: wait for memory
+        q->arena_off = off;
+        q->bv.p.out = reinterpret_cast<uint32_t*>(s->arena_dev + off);
+        s->active.push_back(q);
+        it = s->pending.erase(it);
+    }
+}
+
+uint32_t grid_for(xd_server* s, uint32_t slots) {
+    auto it = s->grid_of_slots.find(slots);
+    if (it != s->grid_of_slots.end()) return it->second;
+    uint32_t g = 0;
+    if (batch_grid(s->ctx->device, slots, s->w8, &g) != XD_OK) return 0;
+    s->grid_of_slots[slots] = g;
+    return g;
+}
+
+// Builds and queues one scheduling step on ring entry `k`.  Returns false if
+// no admitted request has tiles left to schedule.
+bool launch_step(xd_server* s, Step& S, std::string* err) {
+    std::vector<Request*> work;
+    for (Request* q : s->active)
+        if (q->tiles_sched < q->tiles_total) work.push_back(q);
+    if (work.empty()) return false;
+    const uint64_t budget = s->cfg.tiles_per_launch;
+    const uint64_t quota = std::max<uint64_t>(budget / work.size(), 64);
+    S.parts.clear();
+    uint64_t left = budget;
+    std::vector<uint64_t> take(work.size(), 0);
+    for (size_t i = 0; i < work.size() && left; ++i) {
+        const uint64_t k = std::min({work[i]->tiles_total - work[i]->tiles_sched, quota, left});
+        take[i] = k;
+        left -= k;
+    }
+    for (size_t i = 0; i < work.size() && left; ++i) {  // unused share goes to the oldest
+        const uint64_t k = std::min(work[i]->tiles_total - work[i]->tiles_sched - take[i], left);
+        take[i] += k;
+        left -= k;
+    }
+    const double now = ms_since(s->t0);
+    uint32_t slots = 1;
+    BatchView* views = reinterpret_cast<BatchView*>(S.host + s->views_off);
+    uint32_t* tiles = reinterpret_cast<uint32_t*>(S.host + s->tiles_off);
+    uint32_t n = 0;
+    for (size_t i = 0; i < work.size(); ++i) {
+        if (!take[i]) continue;
+        Request* q = work[i];
+        const uint32_t j = (uint32_t)S.parts.size();
+        S.parts.push_back({q, (uint32_t)q->tiles_sched, (uint32_t)take[i]});
+        views[j] = q->bv;
+        slots = std::max(slots, batch_view_slots(q->bv));
+        for (uint64_t t = q->tiles_sched; t < q->tiles_sched + take[i]; ++t) tiles[n++] = (j << 20) | (uint32_t)t;
+        q->tiles_sched += take[i];
+        if (q->c.launches++ == 0) q->c.start_ms = now;
+    }
+    for (size_t j = 0; j < S.parts.size(); ++j) views[j].p.lds_slots = slots;
+    // contiguous runs of the step's tiles, one per XCD
+    uint32_t* qbeg = reinterpret_cast<uint32_t*>(S.host + s->qbeg_off);
+    for (uint32_t k = 0; k <= kQueues; ++k) qbeg[k] = (uint32_t)((uint64_t)n * k / kQueues);
+    const uint32_t grid = grid_for(s, slots);
+    if (!grid) {
+        *err = "served step: occupancy query failed";
+        return false;
+    }
+    const size_t bytes = s->tiles_off + (size_t)n * sizeof(uint32_t);
+    hipError_t e;
+    if ((e = hipMemcpyAsync(S.dev, S.host, bytes, hipMemcpyHostToDevice, S.stream)) != hipSuccess ||
+        (e = hipEventRecord(S.ev0, S.stream)) != hipSuccess ||
+        (e = launch_batch(reinterpret_cast<const BatchView*>(S.dev + s->views_off),
+                          reinterpret_cast<const uint32_t*>(S.dev + s->tiles_off),
+                          reinterpret_cast<const uint32_t*>(S.dev + s->qbeg_off),
+                          reinterpret_cast<uint32_t*>(S.dev + s->counters_off), s->grab, slots, s->w8, grid,
+                          S.stream)) != hipSuccess ||
+        (e = hipEventRecord(S.ev1, S.stream)) != hipSuccess) {
+        *err = std::string("served step launch: ") + hipGetErrorString(e);
+        return false;
+    }
+    S.busy = true;
+    S.seq = ++s->step_seq;
+    s->st.launches++;
+    s->st.tiles += n;
+    s->st.max_requests_per_launch = std::max<uint32_t>(s->st.max_requests_per_launch, (uint32_t)S.parts.size());
+    uint32_t busy = 0;
+    for (uint32_t k = 0; k < s->ring; ++k) busy += s->steps[k].busy;
+    s->st.max_launches_overlapped = std::max(s->st.max_launches_overlapped, busy);
+    return true;
+}
+
+void finish_step(xd_server* s, Step& S) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, S.ev0, S.ev1) == hipSuccess) s->st.busy_ms += ms;
+    const double now = ms_since(s->t0);
+    for (const Part& p : S.parts) {
+        Request* q = p.req;
+        q->tiles_done += p.count;
+        if (q->tiles_done == q->tiles_total) {
+            q->c.device_done_ms = now;
+            s->readq.push_back(q);
+        }
+    }
+    S.parts.clear();
+    S.busy = false;
+    s->cv_read.notify_one();
+}
+
+void worker_main(xd_server* s) {
+    (void)hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        if (!s->broken.empty()) break;
+        const double now = ms_since(s->t0);
+        admit(s, now);
+        // queue steps while ring entries are free
+        std::string err;
+        for (uint32_t k = 0; k < s->ring; ++k) {
+            Step& S = s->steps[k];
+            if (S.busy) continue;
+            if (!launch_step(s, S, &err)) break;
+        }
+        if (!err.empty()) {
+            s->broken = err;
+            break;
+        }
+        // the oldest step in flight
+        Step* oldest = nullptr;
+        for (uint32_t k = 0; k < s->ring; ++k)
+            if (s->steps[k].busy && (!oldest || s->steps[k].seq < oldest->seq)) oldest = &s->steps[k];
+        if (oldest) {
+            const hipError_t q = hipEventQuery(oldest->ev1);
+            if (q == hipSuccess) {
+                finish_step(s, *oldest);
+                continue;
+            }
+            if (q != hipErrorNotReady) {
+                s->broken = std::string("served step: ") + hipGetErrorString(q);
+                break;
+            }
+            s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+            continue;
+        }
+        const bool idle = s->active.empty() && s->readq.empty();
+        if (s->stop && s->pending.empty() && idle) break;
+        // nothing on the device: sleep until a request arrives or is released
+        double next_release = -1;
+        if (!s->pending.empty() && s->pending.begin()->first > now) next_release = s->pending.begin()->first;
+        if (next_release >= 0)
+            s->cv_work.wait_for(lk, std::chrono::duration<double, std::milli>(next_release - now));
+        else
+            s->cv_work.wait(lk);
+    }
+    if (!s->broken.empty()) {
+        // let launched steps drain before the frames they write are dropped
+        for (uint32_t k = 0; k < s->ring; ++k)
+            if (s->steps[k].busy) (void)hipEventSynchronize(s->steps[k].ev1);
+        abort_all(s, XD_ERR_DEVICE);
+    }
+    s->reader_stop = true;
+    s->cv_read.notify_all();
+}
+
+void reader_main(xd_server* s) {
+    (void)hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_read.wait(lk, [&] { return !s->readq.empty() || s->reader_stop; });
+        if (s->readq.empty()) break;
+        Request* q = s->readq.front();
+        s->readq.pop_front();
+        lk.unlock();
+        hipError_t e = hipSuccess;
+        if (q->r.dst) {
+            e = hipMemcpyAsync(q->r.dst, s->arena_dev + q->arena_off, q->frame_bytes, hipMemcpyDeviceToHost,
+                               s->read_stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(s->read_stream);
+        }
+        const double now = ms_since(s->t0);
+        lk.lock();
+        q->c.done_ms = q->r.dst ? now : q->c.device_done_ms;
+        q->c.status = e == hipSuccess ? XD_OK : XD_ERR_DEVICE;
+        s->arena.release(q->arena_off, q->frame_bytes);
+        s->active.erase(std::find(s->active.begin(), s->active.end(), q));
+        s->vols[q->vol_id].refs--;
+        s->st.completed++;
+        s->st.rays += q->c.rays;
+        s->done.push_back(q->c);
+        delete q;
+        s->cv_done.notify_all();
+        s->cv_work.notify_one();
+    }
+}
+
+void free_device(xd_server* s) {
+    for (uint32_t k = 0; k < kMaxRing; ++k) {
+        Step& S = s->steps[k];
+        if (S.stream) (void)hipStreamSynchronize(S.stream);
+        if (S.ev0) (void)hipEventDestroy(S.ev0);
+        if (S.ev1) (void)hipEventDestroy(S.ev1);
+        if (S.stream) (void)hipStreamDestroy(S.stream);
+        if (S.host) (void)hipHostFree(S.host);
+        if (S.dev) (void)hipFree(S.dev);
+        S = Step{};
+    }
+    if (s->read_stream) (void)hipStreamDestroy(s->read_stream);
+    if (s->arena_dev) (void)hipFree(s->arena_dev);
+    s->read_stream = nullptr;
+    s->arena_dev = nullptr;
+}
+
+int load_volume(xd_server* s, int kind, const void* data, uint64_t x, uint64_t y, uint64_t z, uint64_t n,
+                uint64_t side, uint32_t* id) {
+    if (!s || !data || !id) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    uint64_t fp = 0;
+    int rc = xd_volume_footprint(s->ctx, kind, x, y, z, n, &fp);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->resident + fp > s->cfg.hbm_budget_bytes)
+            return fail(XD_ERR_REJECTED, "volume needs %llu bytes of HBM; %llu of the %llu-byte budget are in use",
+                        (unsigned long long)fp, (unsigned long long)s->resident,
+                        (unsigned long long)s->cfg.hbm_budget_bytes);
+        s->resident += fp;  // reserved while the upload runs
+    }
+    xd_volume* v = nullptr;
+    rc = kind == XD_VOLUME_GRID ? xd_volume_from_grid(s->ctx, data, x, y, z, &v)
+                                : xd_volume_from_svo(s->ctx, data, n, side, &v);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (rc) {
+        s->resident -= fp;
+        return rc;
+    }
+    s->resident = s->resident - fp + v->bytes;
+    *id = s->next_vol++;
+    s->vols[*id] = ServedVolume{v, v->bytes, 0};
+    s->st.volumes = (uint32_t)s->vols.size();
+    return XD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** out) {
+    if (!ctx || !out) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    xd_server_config c = cfg ? *cfg : xd_server_config{};
+    if (c.max_active == 0) c.max_active = 256;
+    if (c.max_active > kMaxActive) return fail(XD_ERR_INVALID_ARGUMENT, "max_active above %u", kMaxActive);
+    if (c.frame_arena_bytes == 0) c.frame_arena_bytes = 512ull << 20;
+    if (c.tiles_per_launch == 0) c.tiles_per_launch = 32768;
+    if (c.tiles_per_launch < 64 || c.tiles_per_launch > (1u << 22))
+        return fail(XD_ERR_INVALID_ARGUMENT, "tiles_per_launch must be 64..2^22");
+    if (c.launches_in_flight == 0) c.launches_in_flight = 2;
+    if (c.launches_in_flight > kMaxRing) return fail(XD_ERR_INVALID_ARGUMENT, "launches_in_flight must be 1..4");
+    for (int t = 0; t < 5; ++t) {
+        if (c.tuning.tile_w[t] == 0) c.tuning.tile_w[t] = 8;
+        if (c.tuning.tile_w[t] != 8 && c.tuning.tile_w[t] != 16 && c.tuning.tile_w[t] != 32)
+            return fail(XD_ERR_INVALID_ARGUMENT, "tile width %u is not 8, 16 or 32", c.tuning.tile_w[t]);
+    }
+    if (c.tuning.tiles_per_grab == 0) c.tuning.tiles_per_grab = 1;
+    if (c.tuning.tiles_per_grab > 8) return fail(XD_ERR_INVALID_ARGUMENT, "tiles_per_grab must be 1..8");
+    if (c.tuning.waves_per_simd != 0 && c.tuning.waves_per_simd != 8)
+        return fail(XD_ERR_INVALID_ARGUMENT, "waves_per_simd must be 0 or 8");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    auto* s = new xd_server();
+    s->ctx = ctx;
+    s->t0 = std::chrono::steady_clock::now();
+    for (int t = 0; t < 5; ++t) s->tile_w[t] = c.tuning.tile_w[t];
+    s->grab = c.tuning.tiles_per_grab;
+    s->w8 = c.tuning.waves_per_simd == 8;
+    s->ring = c.launches_in_flight;
+    // table layout of one step: views | qbeg | counters | tiles
+    s->views_off = 0;
+    s->qbeg_off = (sizeof(BatchView) * c.max_active + 255) / 256 * 256;
+    s->counters_off = s->qbeg_off + 64;
+    s->tiles_off = s->counters_off + 64;
+    s->table_bytes = s->tiles_off + (size_t)c.tiles_per_launch * sizeof(uint32_t);
+    if ((e = hipMalloc(&s->arena_dev, c.frame_arena_bytes)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&s->read_stream, hipStreamNonBlocking)) != hipSuccess) {
+        free_device(s);
+        delete s;
+        return fail_hip(e, "server allocation");
+    }
+    for (uint32_t k = 0; k < s->ring; ++k) {
+        Step& S = s->steps[k];
+        if ((e = hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreate(&S.ev0)) != hipSuccess || (e = hipEventCreate(&S.ev1)) != hipSuccess ||
+            (e = hipHostMalloc(reinterpret_cast<void**>(&S.host), s->table_bytes, hipHostMallocDefault)) !=
+                hipSuccess ||
+            (e = hipMalloc(&S.dev, s->table_bytes)) != hipSuccess) {
+            free_device(s);
+            delete s;
+            return fail_hip(e, "server step tables");
+        }
+    }
+    s->arena.reset(c.frame_arena_bytes);
+    if (c.hbm_budget_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        if ((e = hipMemGetInfo(&fr, &tot)) != hipSuccess) {
+            free_device(s);
+            delete s;
+            return fail_hip(e, "hipMemGetInfo");
+        }
+        c.hbm_budget_bytes = (uint64_t)(fr * 0.9);
+    }
+    s->cfg = c;
+    s->st.budget_bytes = c.hbm_budget_bytes;
+    s->worker = std::thread(worker_main, s);
+    s->reader = std::thread(reader_main, s);
+    ctx->servers.insert(s);
+    *out = s;
+    return XD_OK;
+}
+
+int xd_server_load_grid(xd_server* s, const void* rgba8, uint64_t x, uint64_t y, uint64_t z, uint32_t* id) {
+    return load_volume(s, XD_VOLUME_GRID, rgba8, x, y, z, 0, 0, id);
+}
+
+int xd_server_load_svo(xd_server* s, const void* nodes, uint64_t n, uint64_t side, uint32_t* id) {
+    return load_volume(s, XD_VOLUME_SVO, nodes, 0, 0, 0, n, side, id);
+}
+
+int xd_server_unload(xd_server* s, uint32_t id) {
+    if (!s) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    xd_volume* v = nullptr;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->vols.find(id);
+        if (it == s->vols.end()) return fail(XD_ERR_INVALID_ARGUMENT, "no volume %u on this server", id);
+        s->cv_done.wait(lk, [&] { return it->second.refs == 0; });
+        v = it->second.v;
+        s->resident -= it->second.bytes;
+        s->vols.erase(it);
+        s->st.volumes = (uint32_t)s->vols.size();
+    }
+    return xd_volume_free(v);
+}
+
+int xd_server_submit(xd_server* s, const xd_request* req, uint64_t* request_id) {
+    if (!s || !req) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    const xd_request& r = *req;
+    if (r.width == 0 || r.height == 0 || r.width > kMaxRequestSide || r.height > kMaxRequestSide)
+        return fail(XD_ERR_INVALID_ARGUMENT, "request size %ux%u is not in 1..%u", r.width, r.height, kMaxRequestSide);
+    auto* q = new Request();
+    q->r = r;
+    q->vol_id = r.volume_id;
+    q->frame_bytes = (uint64_t)r.width * r.height * sizeof(uint32_t);
+    const double now = ms_since(s->t0);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->broken.empty()) {
+        delete q;
+        return fail(XD_ERR_DEVICE, "server stopped after a device error: %s", s->broken.c_str());
+    }
+    if (s->stop) {
+        delete q;
+        return fail(XD_ERR_INVALID_ARGUMENT, "server is shutting down");
+    }
+    auto it = s->vols.find(r.volume_id);
+    if (it == s->vols.end()) {
+        delete q;
+        return fail(XD_ERR_INVALID_ARGUMENT, "no volume %u on this server", r.volume_id);
+    }
+    const xd_volume* vol = it->second.v;
+    int rc = check_traversal(r.traversal, vol);
+    if (rc) {
+        delete q;
+        return rc;
+    }
+    if (q->frame_bytes > s->cfg.frame_arena_bytes) {
+        delete q;
+        return fail(XD_ERR_REJECTED, "a %ux%u frame never fits the %llu-byte frame arena", r.width, r.height,
+                    (unsigned long long)s->cfg.frame_arena_bytes);
+    }
+    xd_uniforms u{};
+    u.output_region.extent[0] = u.display_region.extent[0] = r.width;
+    u.output_region.extent[1] = u.display_region.extent[1] = r.height;
+    for (int i = 0; i < 3; ++i) {
+        u.voxel_ratio[i] = r.voxel_ratio[i] == 0.0f ? 1.0f : r.voxel_ratio[i];
+        u.model_dim[i] = (uint32_t)(vol->kind == XD_VOLUME_GRID ? vol->dims[i] : vol->side);
+    }
+    u.emission_coeff = r.emission_coeff;
+    make_batch_view(vol, r.traversal, &r.camera, &u, nullptr, r.width, s->tile_w[r.traversal], q->bv, &q->tiles_total);
+    if (q->tiles_total > kMaxTilesPerView) {
+        delete q;
+        return fail(XD_ERR_INVALID_ARGUMENT, "request has %llu tiles (limit %u)", (unsigned long long)q->tiles_total,
+                    kMaxTilesPerView);
+    }
+    q->id = s->next_id++;
+    q->c.request_id = q->id;
+    q->c.user_tag = r.user_tag;
+    q->c.arrival_ms = r.release_ms > 0 ? r.release_ms : now;
+    q->c.rays = (uint64_t)r.width * r.height;
+    it->second.refs++;
+    s->pending.emplace(q->c.arrival_ms, q);
+    if (request_id) *request_id = q->id;
+    s->cv_work.notify_one();
+    return XD_OK;
+}
+
+int xd_server_poll(xd_server* s, xd_completion* out, uint32_t max, uint32_t* n, int32_t timeout_ms) {
+    if (!s || !n || (!out && max)) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    *n = 0;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto ready = [&] { return !s->done.empty(); };
+    if (!ready() && timeout_ms < 0) {
+        if (s->pending.empty() && s->active.empty() && s->readq.empty())
+            return fail(XD_ERR_INVALID_ARGUMENT, "nothing to wait for: no request is queued or running");
+        s->cv_done.wait(lk, ready);
+    } else if (!ready() && timeout_ms > 0) {
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+    }
+    while (*n < max && !s->done.empty()) {
+        out[(*n)++] = s->done.front();
+        s->done.pop_front();
+    }
+    return XD_OK;
+}
+
+int xd_server_stats_get(xd_server* s, xd_server_stats* out) {
+    if (!s || !out) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->st;
+    out->resident_bytes = s->resident;
+    out->pending = (uint32_t)s->pending.size();
+    out->active = (uint32_t)s->active.size();
+    return XD_OK;
+}
+
+double xd_server_now_ms(const xd_server* s) { return s ? ms_since(s->t0) : 0.0; }
+
+int xd_server_free(xd_server* s) {
+    if (!s) return XD_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_work.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    if (s->reader.joinable()) s->reader.join();
+    (void)hipSetDevice(s->ctx->device);
+    free_device(s);
+    for (auto& kv : s->vols) (void)xd_volume_free(kv.second.v);
+    for (auto& kv : s->pending) delete kv.second;
+    s->ctx->servers.erase(s);
+    delete s;
+    return XD_OK;
+}
+
+}  // extern "C"
