@@ -80,7 +80,8 @@ class BuildStats(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("frame_arena_bytes", ctypes.c_uint64),
                 ("max_active", ctypes.c_uint32), ("tiles_per_launch", ctypes.c_uint32),
-                ("launches_in_flight", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("tuning", BatchTuning)]
+                ("launches_in_flight", ctypes.c_uint32), ("readback_threads", ctypes.c_uint32),
+                ("tuning", BatchTuning)]
 
 
 class Request(ctypes.Structure):
@@ -262,13 +263,14 @@ class Context:
         return self._adopt(Batch(self, h))
 
     def server(self, hbm_budget_bytes=0, frame_arena_bytes=0, max_active=0, tiles_per_launch=0,
-               launches_in_flight=0, tuning=None):
+               launches_in_flight=0, readback_threads=0, tuning=None):
         c = ServerConfig()
         c.hbm_budget_bytes = hbm_budget_bytes
         c.frame_arena_bytes = frame_arena_bytes
         c.max_active = max_active
         c.tiles_per_launch = tiles_per_launch
         c.launches_in_flight = launches_in_flight
+        c.readback_threads = readback_threads
         if tuning:
             c.tuning.tile_w[:] = list(tuning.get("tile_w", [8] * 5))
             c.tuning.tiles_per_grab = tuning.get("tiles_per_grab", 1)

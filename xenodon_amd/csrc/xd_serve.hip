@@ -12,8 +12,9 @@
 //              own view/tile tables), so requests that arrive while others
 //              run are folded into the next step and their tiles overlap on
 //              the device with the tail of the running one.
-//   reader     copies finished frames to the callers' host buffers and
-//              releases the request's slot and arena space.
+//   readers    copy finished frames to the callers' host buffers (each on
+//              its own stream) and release the request's slot and arena
+//              space.
 //
 // A tile is shaded by the same per-ray code as the plain and batched forms,
 // so a result is bit-identical however it was batched.
@@ -143,7 +144,7 @@ struct xd_server {
     // device resources
     uint8_t* arena_dev = nullptr;
     Arena arena;
-    hipStream_t read_stream = nullptr;
+    std::vector<hipStream_t> read_streams;
     Step steps[kMaxRing];
     uint32_t ring = 2;
     uint64_t step_seq = 0;
@@ -158,7 +159,8 @@ struct xd_server {
     std::string broken;  // first device error; the server then refuses work
     xd_server_stats st{};
 
-    std::thread worker, reader;
+    std::thread worker;
+    std::vector<std::thread> readers;
 };
 
 namespace {
@@ -211,8 +213,8 @@ uint32_t grid_for(xd_server* s, uint32_t slots) {
     return g;
 }
 
-// Builds and queues one scheduling step on ring entry `k`.  Returns false if
-// no admitted request has tiles left to schedule.
+// Builds and queues one scheduling step on the free ring entry S.  Returns
+// false if no admitted request has tiles left to schedule.
 bool launch_step(xd_server* s, Step& S, std::string* err) {
     std::vector<Request*> work;
     for (Request* q : s->active)
@@ -296,7 +298,7 @@ void finish_step(xd_server* s, Step& S) {
     }
     S.parts.clear();
     S.busy = false;
-    s->cv_read.notify_one();
+    s->cv_read.notify_all();
 }
 
 void worker_main(xd_server* s) {
@@ -354,8 +356,9 @@ void worker_main(xd_server* s) {
     s->cv_read.notify_all();
 }
 
-void reader_main(xd_server* s) {
+void reader_main(xd_server* s, uint32_t idx) {
     (void)hipSetDevice(s->ctx->device);
+    hipStream_t rs = s->read_streams[idx];
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
         s->cv_read.wait(lk, [&] { return !s->readq.empty() || s->reader_stop; });
@@ -366,8 +369,8 @@ void reader_main(xd_server* s) {
         hipError_t e = hipSuccess;
         if (q->r.dst) {
             e = hipMemcpyAsync(q->r.dst, s->arena_dev + q->arena_off, q->frame_bytes, hipMemcpyDeviceToHost,
-                               s->read_stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(s->read_stream);
+                               rs);
+            if (e == hipSuccess) e = hipStreamSynchronize(rs);
         }
         const double now = ms_since(s->t0);
         lk.lock();
@@ -396,9 +399,10 @@ void free_device(xd_server* s) {
         if (S.dev) (void)hipFree(S.dev);
         S = Step{};
     }
-    if (s->read_stream) (void)hipStreamDestroy(s->read_stream);
+    for (hipStream_t rs : s->read_streams)
+        if (rs) (void)hipStreamDestroy(rs);
+    s->read_streams.clear();
     if (s->arena_dev) (void)hipFree(s->arena_dev);
-    s->read_stream = nullptr;
     s->arena_dev = nullptr;
 }
 
@@ -447,6 +451,8 @@ int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** o
         return fail(XD_ERR_INVALID_ARGUMENT, "tiles_per_launch must be 64..2^22");
     if (c.launches_in_flight == 0) c.launches_in_flight = 2;
     if (c.launches_in_flight > kMaxRing) return fail(XD_ERR_INVALID_ARGUMENT, "launches_in_flight must be 1..4");
+    if (c.readback_threads == 0) c.readback_threads = 4;
+    if (c.readback_threads > 16) return fail(XD_ERR_INVALID_ARGUMENT, "readback_threads must be 1..16");
     for (int t = 0; t < 5; ++t) {
         if (c.tuning.tile_w[t] == 0) c.tuning.tile_w[t] = 8;
         if (c.tuning.tile_w[t] != 8 && c.tuning.tile_w[t] != 16 && c.tuning.tile_w[t] != 32)
@@ -471,8 +477,11 @@ int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** o
     s->counters_off = s->qbeg_off + 64;
     s->tiles_off = s->counters_off + 64;
     s->table_bytes = s->tiles_off + (size_t)c.tiles_per_launch * sizeof(uint32_t);
-    if ((e = hipMalloc(&s->arena_dev, c.frame_arena_bytes)) != hipSuccess ||
-        (e = hipStreamCreateWithFlags(&s->read_stream, hipStreamNonBlocking)) != hipSuccess) {
+    s->read_streams.assign(c.readback_threads, nullptr);
+    e = hipMalloc(&s->arena_dev, c.frame_arena_bytes);
+    for (uint32_t k = 0; k < c.readback_threads && e == hipSuccess; ++k)
+        e = hipStreamCreateWithFlags(&s->read_streams[k], hipStreamNonBlocking);
+    if (e != hipSuccess) {
         free_device(s);
         delete s;
         return fail_hip(e, "server allocation");
@@ -502,7 +511,7 @@ int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** o
     s->cfg = c;
     s->st.budget_bytes = c.hbm_budget_bytes;
     s->worker = std::thread(worker_main, s);
-    s->reader = std::thread(reader_main, s);
+    for (uint32_t k = 0; k < c.readback_threads; ++k) s->readers.emplace_back(reader_main, s, k);
     ctx->servers.insert(s);
     *out = s;
     return XD_OK;
@@ -632,7 +641,8 @@ int xd_server_free(xd_server* s) {
         s->cv_work.notify_all();
     }
     if (s->worker.joinable()) s->worker.join();
-    if (s->reader.joinable()) s->reader.join();
+    for (auto& r : s->readers)
+        if (r.joinable()) r.join();
     (void)hipSetDevice(s->ctx->device);
     free_device(s);
     for (auto& kv : s->vols) (void)xd_volume_free(kv.second.v);

@@ -147,72 +147,95 @@ __device__ __forceinline__ Planes node_planes(f3 corner, float side, f3 rrd, f3 
     return P;
 }
 
-// svo_df.comp:6-66.  The shader's walk (children in index order, stack of
-// (node, child index), tail call for child 7) with the per-child box taken
-// from the node's planes.  The child position the shader carries in `pos`
-// is recomputed where it is needed: on descent (corner + side on the set
-// axes) and on pop (the shader's pos - mod(pos, 2 * side) from the position
-// of child 7, corner + side).
+// Bit k set when child box k of the node is hit (t_min < t_max && t_max > 0,
+// svo_df.comp:25-32).  Per axis the low child spans [T0, T1] and the high
+// child [T1, T2]; min/max of finite floats are exact, so these are the same
+// comparisons as the shader's per-child test.
+__device__ __forceinline__ uint32_t child_hits(const Planes& P) {
+    const float xa0 = gmin(P.x0, P.x1), xa1 = gmax(P.x0, P.x1), xb0 = gmin(P.x1, P.x2), xb1 = gmax(P.x1, P.x2);
+    const float ya0 = gmin(P.y0, P.y1), ya1 = gmax(P.y0, P.y1), yb0 = gmin(P.y1, P.y2), yb1 = gmax(P.y1, P.y2);
+    const float za0 = gmin(P.z0, P.z1), za1 = gmax(P.z0, P.z1), zb0 = gmin(P.z1, P.z2), zb1 = gmax(P.z1, P.z2);
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+        const float lo = gmax((k & 4u) ? xb0 : xa0, gmax((k & 2u) ? yb0 : ya0, (k & 1u) ? zb0 : za0));
+        const float hi = gmin((k & 4u) ? xb1 : xa1, gmin((k & 2u) ? yb1 : ya1, (k & 1u) ? zb1 : za1));
+        m |= (lo < hi && hi > 0.0f) ? (1u << k) : 0u;
+    }
+    return m;
+}
+
+// svo_df.comp:6-66.  The shader walks the children of a node in index order,
+// testing each box, and keeps (node, child index) on a stack.  Here a node's
+// eight box tests are done at once into a hit mask, and only hit children are
+// visited -- still in index order, so leaves are summed in the shader's order
+// with the shader's t values.  A stack entry is (node, children still to
+// visit); a node with none left is not pushed (the shader pops it and finds
+// nothing more to do).  The child position the shader carries in `pos` is
+// recomputed where it is needed: on descent (corner + side on the set axes)
+// and on pop (the shader's pos - mod(pos, 2 * side) from a point inside the
+// node being finished, which lies inside every ancestor).
 __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t slots, f3 ro, f3 rd,
                                       uint32_t* stk_node, uint32_t* stk_child) {
     f3 rrd = divv(splat(1.0f), rd);
     f3 bias = mul(rrd, ro);
     const int cap_sp = (int)slots;
     int sp = 0;
-    uint32_t node = 0, child_idx = 0;
+    uint32_t node = 0;
     f3 corner = splat(0.0f);
     float side = 0.5f;
     f3 total = splat(0.0f);
     uint4 lo, hi;
     load8(s, node, lo, hi);
     Planes P = node_planes(corner, side, rrd, bias);
+    uint32_t mask = child_hits(P);
     const uint32_t cap32 = cap > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)cap;
     uint32_t it = 0;
     for (;;) {
-        if (++it > cap32) break;
-        const uint32_t wd = pick8(lo, hi, child_idx);
-        const bool bx = (child_idx & 4u) != 0, by = (child_idx & 2u) != 0, bz = (child_idx & 1u) != 0;
-        const float mnx = bx ? P.x1 : P.x0, mxx = bx ? P.x2 : P.x1;
-        const float mny = by ? P.y1 : P.y0, mxy = by ? P.y2 : P.y1;
-        const float mnz = bz ? P.z1 : P.z0, mxz = bz ? P.z2 : P.z1;
-        const float t_min = max_elem(mk(gmin(mnx, mxx), gmin(mny, mxy), gmin(mnz, mxz)));
-        const float t_max = min_elem(mk(gmax(mnx, mxx), gmax(mny, mxy), gmax(mnz, mxz)));
-        if (t_min < t_max && t_max > 0.0f) {
-            const uint32_t child = wd & kIndexMask;
-            if (wd & kTagLeaf) {
-                total = add(total, muls(unorm_rgb(colour_if(s, child, wd)), t_max - gmax(t_min, 0.0f)));
-            } else if (child_idx == 7 || sp < cap_sp) {
-                if (child_idx != 7) {
-                    stk_node[sp * kWaveLanes] = node;
-                    stk_child[sp * kWaveLanes] = child_idx;
-                    ++sp;
-                }
-                corner = mk(bx ? corner.x + side : corner.x, by ? corner.y + side : corner.y,
-                            bz ? corner.z + side : corner.z);
-                side = side * 0.5f;
-                node = child;
-                child_idx = 0;
-                load8(s, node, lo, hi);
-                P = node_planes(corner, side, rrd, bias);
-                continue;
-            }
-        }
-        if (child_idx == 7) {
+        if (mask == 0) {
+            if (sp == 0) break;
             --sp;
-            if (sp < 0) break;
             node = min(stk_node[sp * kWaveLanes], s.n - 1u);
-            child_idx = stk_child[sp * kWaveLanes] & 7u;
+            mask = stk_child[sp * kWaveLanes] & 0xFFu;
             load8(s, node, lo, hi);
             const int d = (int)(s.meta[node].y & kDepthMask);
-            const f3 pos = mk(corner.x + side, corner.y + side, corner.z + side);  // child 7
+            const f3 pos = mk(corner.x + side, corner.y + side, corner.z + side);
             side = __builtin_ldexpf(1.0f, -d) * 0.5f;
             // side is a power of two, so mod(pos, 2 side) is exact with the reciprocal
             const float s2 = side * 2.0f;
             const float inv = __builtin_ldexpf(1.0f, -__builtin_amdgcn_frexp_expf(s2) + 1);
             corner = sub(pos, mk(pmod(pos.x, s2, inv), pmod(pos.y, s2, inv), pmod(pos.z, s2, inv)));
             P = node_planes(corner, side, rrd, bias);
+            if (mask == 0) continue;
         }
-        ++child_idx;
+        if (++it > cap32) break;
+        const uint32_t ci = (uint32_t)__builtin_ctz(mask);
+        mask &= mask - 1u;
+        const uint32_t wd = pick8(lo, hi, ci);
+        const bool bx = (ci & 4u) != 0, by = (ci & 2u) != 0, bz = (ci & 1u) != 0;
+        if (wd & kTagLeaf) {
+            if (!(wd & kTagEmpty)) {  // an empty leaf adds exactly +0
+                const float mnx = bx ? P.x1 : P.x0, mxx = bx ? P.x2 : P.x1;
+                const float mny = by ? P.y1 : P.y0, mxy = by ? P.y2 : P.y1;
+                const float mnz = bz ? P.z1 : P.z0, mxz = bz ? P.z2 : P.z1;
+                const float t_min = max_elem(mk(gmin(mnx, mxx), gmin(mny, mxy), gmin(mnz, mxz)));
+                const float t_max = min_elem(mk(gmax(mnx, mxx), gmax(mny, mxy), gmax(mnz, mxz)));
+                total = add(total, muls(unorm_rgb(s.meta[wd & kIndexMask].x), t_max - gmax(t_min, 0.0f)));
+            }
+        } else if (mask == 0 || sp < cap_sp) {
+            if (mask != 0) {
+                stk_node[sp * kWaveLanes] = node;
+                stk_child[sp * kWaveLanes] = mask;
+                ++sp;
+            }
+            corner = mk(bx ? corner.x + side : corner.x, by ? corner.y + side : corner.y,
+                        bz ? corner.z + side : corner.z);
+            side = side * 0.5f;
+            node = wd & kIndexMask;
+            load8(s, node, lo, hi);
+            P = node_planes(corner, side, rrd, bias);
+            mask = child_hits(P);
+        }
     }
     return total;
 }
