@@ -91,7 +91,8 @@ typedef struct {
     uint64_t device_bytes; /* HBM held by this volume */
     uint32_t is_rope;      /* leaves carry ropes (all leaf children[6..7]==0 and
                               some rope != 0) */
-    uint32_t reserved;
+    uint32_t depth_fields_exact; /* SVO: every internal node's depth field equals
+                                    its depth in the tree (converter output) */
 } xd_volume_info;
 
 /* ---- errors / devices --------------------------------------------------- */

@@ -216,6 +216,29 @@ def test_raw_octree_layout_bit_exact(t):
 
 
 @pytest.mark.parametrize("t", [abi.SVO_NAIVE, abi.ESVO, abi.SVO_DF, abi.SVO_ROPE])
+def test_tree_with_inexact_depth_fields_bit_exact(ctx, t):
+    """svo-df sizes a popped node from its depth field (svo_df.comp:58) and
+    svo-rope sizes a rope target the same way (svo_rope.comp:30); a tree whose
+    internal depth fields are off renders exactly as the shaders would, and
+    the device flags it so no depth is taken from the tree's structure."""
+    g = H.blocky_grid(32, 2, seed=17)
+    nodes, st = xo.build_octree(g, tree=xo.TREE_ROPE)
+    bad = nodes.copy()
+    internal = np.nonzero((bad[:, 9] & 0x80000000) == 0)[0][1:]  # all but the root
+    assert internal.size > 4
+    bad[internal[::2], 9] += 1
+    good_v = ctx.svo_volume(nodes, st["side"])
+    bad_v = ctx.svo_volume(bad, st["side"])
+    assert good_v.info().depth_fields_exact == 1 and bad_v.info().depth_fields_exact == 0
+    ov = xo.OracleVolume(nodes=bad, side=st["side"])
+    for k, (f, up, eye) in enumerate(H.orbit_cameras(3, 1.6, seed=12)):
+        args = (64, 48, [st["side"]] * 3)
+        img, _ = ctx.render_image(bad_v, t, abi.make_camera(f, up, eye), abi.make_uniforms(*args, emission=2.0))
+        ref = xo.render(t, ov, xo.make_camera(f, up, eye), xo.make_uniforms(*args, emission=2.0))["rgba"]
+        _assert_parity(img, ref, f"inexact-depth/{abi.TRAVERSAL_NAMES[t]}/cam{k}")
+
+
+@pytest.mark.parametrize("t", [abi.SVO_NAIVE, abi.ESVO, abi.SVO_DF, abi.SVO_ROPE])
 def test_dag_and_single_leaf_trees_bit_exact(ctx, t):
     """--dag trees share subtrees between parents (the device renumbering
     keeps a shared node's first index); a uniform volume is one leaf root."""

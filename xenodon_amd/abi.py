@@ -58,7 +58,7 @@ class VolumeInfo(ctypes.Structure):
         ("node_count", ctypes.c_uint64),
         ("device_bytes", ctypes.c_uint64),
         ("is_rope", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("depth_fields_exact", ctypes.c_uint32),
     ]
 
 

@@ -2,6 +2,7 @@
 
   xenodon_amd/libxenodon_amd.so   HIP kernels + C-ABI (include/xenodon.h), gfx950
   xenodon_amd/bin/xenodon         C++ CLI (render / convert / help / sysinfo)
+  xenodon_amd/libxd_io.so         the CLI's file formats (TIFF, .svo, PNG) as a C library
 
 Run `python -m xenodon_amd.build` (or __graft_entry__.build()).  hipcc
 cross-compiles for gfx950 without a GPU.
@@ -74,9 +75,42 @@ def build_synth():
     return SYNTH_LIB
 
 
+CLI_SOURCES = ["main.cpp", "cli_util.cpp", "volume_io.cpp"]
+CLI = os.path.join(BIN_DIR, "xenodon")
+
+
+def build_cli():
+    """The xenodon CLI (host C++), linked against the library in-tree."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    cli_dir = os.path.join(PKG, "cli")
+    tmp = CLI + ".tmp"
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-Wno-unused-parameter", "-I", INCLUDE,
+         *[os.path.join(cli_dir, s) for s in CLI_SOURCES], "-o", tmp, "-L", PKG, "-lxenodon_amd", "-lz",
+         f"-Wl,-rpath-link,{rocm_lib}", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(tmp, CLI)
+    return CLI
+
+
+IO_LIB = os.path.join(PKG, "libxd_io.so")
+
+
+def build_io():
+    """The CLI's file formats as a C library (xenodon_amd/io.py)."""
+    cli_dir = os.path.join(PKG, "cli")
+    tmp = IO_LIB + ".tmp"
+    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-Wall", "-I", INCLUDE,
+         *[os.path.join(cli_dir, s) for s in ("volume_io.cpp", "cli_util.cpp", "volume_io_capi.cpp")],
+         "-o", tmp, "-lz"])
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
+
+
 def build_all():
     build_library()
     build_synth()
+    build_cli()
+    build_io()
     return LIB
 
 

@@ -93,4 +93,29 @@ void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint3
     for (auto& x : th) x.join();
 }
 
+bool octree_depths_exact(const uint32_t* w, uint64_t n) {
+    std::vector<uint8_t> depth(n, 0xFF);
+    std::vector<uint32_t> level{0}, next;
+    depth[0] = 0;
+    for (uint32_t d = 0; !level.empty(); ++d) {
+        next.clear();
+        for (uint32_t i : level) {
+            const uint32_t* nd = w + (uint64_t)i * kNodeWords;
+            if (nd[9] & kLeafMask) continue;
+            if ((nd[9] & kDepthMask) != d) return false;
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t c = nd[k];
+                if (depth[c] == 0xFF) {
+                    depth[c] = (uint8_t)std::min<uint32_t>(d + 1, 0xFE);
+                    next.push_back(c);
+                } else if (depth[c] != d + 1) {
+                    return false;
+                }
+            }
+        }
+        level.swap(next);
+    }
+    return true;
+}
+
 }  // namespace xd

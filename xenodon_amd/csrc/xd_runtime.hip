@@ -109,6 +109,7 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
         p.stack_slots = vol->max_depth + 1;
         p.lds_slots = p.stack_slots;
         p.node_count = (uint32_t)vol->node_count;
+        p.depth_exact = vol->depth_exact;
     }
 }
 
@@ -403,6 +404,7 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
             return fail(XD_ERR_OUT_OF_MEMORY, "out of host memory deriving the octree layout");
         }
         build_tagged_layout(w, node_count, cw.data(), meta.data(), v->top);
+        v->depth_exact = octree_depths_exact(w, node_count) ? 1u : 0u;
         if ((e = hipMalloc(&v->d_cw, node_count * 8 * sizeof(uint32_t))) != hipSuccess ||
             (e = hipMalloc(&v->d_meta, node_count * sizeof(uint2))) != hipSuccess ||
             (e = hipMemcpy(v->d_cw, cw.data(), node_count * 8 * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
@@ -456,6 +458,7 @@ int xd_volume_info_get(const xd_volume* v, xd_volume_info* out) {
     out->node_count = v->node_count;
     out->device_bytes = v->bytes;
     out->is_rope = v->is_rope;
+    out->depth_fields_exact = v->depth_exact;
     return XD_OK;
 }
 

@@ -46,6 +46,7 @@ struct xd_volume {
     uint32_t* d_cw = nullptr;
     uint2* d_meta = nullptr;
     uint32_t root_tag = 0;
+    uint32_t depth_exact = 0;  // internal nodes' depth fields match their depth (see octree_depths_exact)
     uint32_t top[xd::kTopLevels] = {0, 0, 0, 0};  // nodes in breadth-first levels 0..L
 };
 
@@ -74,5 +75,10 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
 // (child_w: n*8 words, meta: n*2 words) from .svo node words.
 // top[L] = number of nodes in levels 0..L, which hold indices [0, top[L]).
 void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels]);
+// True when every internal node reachable from the root is reached only at
+// one depth and its is_leaf_depth depth field equals that depth (root = 0),
+// as in every tree the converter writes.  Traversals may then track depth
+// themselves instead of reading the field back.
+bool octree_depths_exact(const uint32_t* w, uint64_t n);
 
 }  // namespace xd

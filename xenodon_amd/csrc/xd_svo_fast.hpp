@@ -24,6 +24,7 @@ struct FastSvo {
     const uint32_t* top;   // LDS copy of the child words of nodes [0, n_top)
     uint32_t n_top;
     uint32_t n;            // node count (indices read back from a stack are clamped to it)
+    uint32_t depth_exact;  // depth fields of internal nodes equal their tree depth
 };
 
 // Child word k of node n: from LDS for the breadth-first top levels.
@@ -170,18 +171,24 @@ __device__ __forceinline__ uint32_t child_hits(const Planes& P) {
 // eight box tests are done at once into a hit mask, and only hit children are
 // visited -- still in index order, so leaves are summed in the shader's order
 // with the shader's t values.  A stack entry is (node, children still to
-// visit); a node with none left is not pushed (the shader pops it and finds
-// nothing more to do).  The child position the shader carries in `pos` is
-// recomputed where it is needed: on descent (corner + side on the set axes)
-// and on pop (the shader's pos - mod(pos, 2 * side) from a point inside the
-// node being finished, which lies inside every ancestor).
+// visit, depth); a node with none left is not pushed (the shader pops it and
+// finds nothing more to do).  Each iteration visits one hit child and then
+// enters at most one node -- the child (descent) or the nearest ancestor with
+// children left (pop) -- through one shared path, so lanes that descend and
+// lanes that pop wait on the same loads.  The child position the shader
+// carries in `pos` is recomputed where it is needed: on descent (corner +
+// side on the set axes) and on pop (the shader's pos - mod(pos, 2 * side),
+// svo_df.comp:50-61, from a point inside the node being finished, which lies
+// inside every ancestor).  The popped node's depth is the depth field the
+// shader reads (svo_df.comp:58); when every depth field equals the node's
+// depth in the tree, the depth kept on the stack is that same value.
 __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t slots, f3 ro, f3 rd,
                                       uint32_t* stk_node, uint32_t* stk_child) {
     f3 rrd = divv(splat(1.0f), rd);
     f3 bias = mul(rrd, ro);
     const int cap_sp = (int)slots;
     int sp = 0;
-    uint32_t node = 0;
+    uint32_t node = 0, depth = 0;
     f3 corner = splat(0.0f);
     float side = 0.5f;
     f3 total = splat(0.0f);
@@ -192,50 +199,57 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
     const uint32_t cap32 = cap > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)cap;
     uint32_t it = 0;
     for (;;) {
-        if (mask == 0) {
+        bool pop = true;
+        if (mask != 0) {
+            if (++it > cap32) break;
+            const uint32_t ci = (uint32_t)__builtin_ctz(mask);
+            mask &= mask - 1u;
+            const uint32_t wd = pick8(lo, hi, ci);
+            const bool bx = (ci & 4u) != 0, by = (ci & 2u) != 0, bz = (ci & 1u) != 0;
+            if (wd & kTagLeaf) {
+                if (!(wd & kTagEmpty)) {  // an empty leaf adds exactly +0
+                    const float mnx = bx ? P.x1 : P.x0, mxx = bx ? P.x2 : P.x1;
+                    const float mny = by ? P.y1 : P.y0, mxy = by ? P.y2 : P.y1;
+                    const float mnz = bz ? P.z1 : P.z0, mxz = bz ? P.z2 : P.z1;
+                    const float t_min = max_elem(mk(gmin(mnx, mxx), gmin(mny, mxy), gmin(mnz, mxz)));
+                    const float t_max = min_elem(mk(gmax(mnx, mxx), gmax(mny, mxy), gmax(mnz, mxz)));
+                    total = add(total, muls(unorm_rgb(s.meta[wd & kIndexMask].x), t_max - gmax(t_min, 0.0f)));
+                }
+                if (mask != 0) continue;  // next sibling, from registers
+            } else if (mask == 0 || sp < cap_sp) {
+                if (mask != 0) {
+                    stk_node[sp * kWaveLanes] = node;
+                    stk_child[sp * kWaveLanes] = mask | (depth << 8);
+                    ++sp;
+                }
+                corner = mk(bx ? corner.x + side : corner.x, by ? corner.y + side : corner.y,
+                            bz ? corner.z + side : corner.z);
+                side = side * 0.5f;
+                node = wd & kIndexMask;
+                depth = depth + 1u;
+                pop = false;
+            } else if (mask != 0) {
+                continue;  // stack full: the shader skips this child
+            }
+        }
+        if (pop) {
             if (sp == 0) break;
             --sp;
             node = min(stk_node[sp * kWaveLanes], s.n - 1u);
-            mask = stk_child[sp * kWaveLanes] & 0xFFu;
-            load8(s, node, lo, hi);
-            const int d = (int)(s.meta[node].y & kDepthMask);
+            const uint32_t e = stk_child[sp * kWaveLanes];
+            mask = e & 0xFFu;
+            depth = (e >> 8) & 0xFFu;
+            const uint32_t d = s.depth_exact ? depth : (s.meta[node].y & kDepthMask);
             const f3 pos = mk(corner.x + side, corner.y + side, corner.z + side);
-            side = __builtin_ldexpf(1.0f, -d) * 0.5f;
+            side = __builtin_ldexpf(1.0f, -(int)d) * 0.5f;
             // side is a power of two, so mod(pos, 2 side) is exact with the reciprocal
             const float s2 = side * 2.0f;
             const float inv = __builtin_ldexpf(1.0f, -__builtin_amdgcn_frexp_expf(s2) + 1);
             corner = sub(pos, mk(pmod(pos.x, s2, inv), pmod(pos.y, s2, inv), pmod(pos.z, s2, inv)));
-            P = node_planes(corner, side, rrd, bias);
-            if (mask == 0) continue;
         }
-        if (++it > cap32) break;
-        const uint32_t ci = (uint32_t)__builtin_ctz(mask);
-        mask &= mask - 1u;
-        const uint32_t wd = pick8(lo, hi, ci);
-        const bool bx = (ci & 4u) != 0, by = (ci & 2u) != 0, bz = (ci & 1u) != 0;
-        if (wd & kTagLeaf) {
-            if (!(wd & kTagEmpty)) {  // an empty leaf adds exactly +0
-                const float mnx = bx ? P.x1 : P.x0, mxx = bx ? P.x2 : P.x1;
-                const float mny = by ? P.y1 : P.y0, mxy = by ? P.y2 : P.y1;
-                const float mnz = bz ? P.z1 : P.z0, mxz = bz ? P.z2 : P.z1;
-                const float t_min = max_elem(mk(gmin(mnx, mxx), gmin(mny, mxy), gmin(mnz, mxz)));
-                const float t_max = min_elem(mk(gmax(mnx, mxx), gmax(mny, mxy), gmax(mnz, mxz)));
-                total = add(total, muls(unorm_rgb(s.meta[wd & kIndexMask].x), t_max - gmax(t_min, 0.0f)));
-            }
-        } else if (mask == 0 || sp < cap_sp) {
-            if (mask != 0) {
-                stk_node[sp * kWaveLanes] = node;
-                stk_child[sp * kWaveLanes] = mask;
-                ++sp;
-            }
-            corner = mk(bx ? corner.x + side : corner.x, by ? corner.y + side : corner.y,
-                        bz ? corner.z + side : corner.z);
-            side = side * 0.5f;
-            node = wd & kIndexMask;
-            load8(s, node, lo, hi);
-            P = node_planes(corner, side, rrd, bias);
-            mask = child_hits(P);
-        }
+        load8(s, node, lo, hi);
+        P = node_planes(corner, side, rrd, bias);
+        if (!pop) mask = child_hits(P);
     }
     return total;
 }
