@@ -1,17 +1,21 @@
-// Batched form: one persistent launch drains a global ray-tile queue that
-// spans many views (any mix of sizes, volumes and traversals).
+// Batched form: one graph launch drains a global ray-tile queue that spans
+// many views (any mix of sizes, volumes and traversals).
 //
 // Queue: every view is cut into wavefront tiles (8x8, 16x4 or 32x2 pixels,
 // chosen per traversal); a tile is one u32 (view << 20 | tile).  Tiles are
-// dealt into 8 ranges by the band of the view they lie in, one range per XCD
-// (the hardware deals workgroups to the XCDs round-robin, so block b runs on
-// XCD b % 8).  A wavefront takes tiles from its own XCD's range with one
-// atomic per grab, and when that range is exhausted it moves on to the other
-// ranges, so the launch ends only when every tile is done and no CU idles
-// while work is left.  Each tile is rendered by exactly the per-ray code of
-// the plain kernels, so every pixel is identical to the plain form.
+// grouped by traversal class, one persistent kernel per class, so that each
+// kernel has the registers and LDS of its own traversal only (a kernel that
+// switches over all five is sized by the worst of them).  Within a class the
+// tiles are dealt into 8 ranges by the band of the view they lie in, one
+// range per XCD (the hardware deals workgroups to the XCDs round-robin, so
+// block b runs on XCD b % 8).  A wavefront takes tiles from its own XCD's
+// range with one atomic per grab, and when that range is exhausted it moves
+// on to the other ranges, so a kernel ends only when every tile of its class
+// is done and no CU idles while work is left.  Each tile is rendered by
+// exactly the per-ray code of the plain kernels, so every pixel is identical
+// to the plain form.
 //
-// The launch (counter reset + persistent kernel) is captured once into a
+// The launch (counter reset + the class kernels) is captured once into a
 // hipGraph and replayed for every batch; new cameras or views only rewrite
 // the device-side view and tile tables.
 #include <hip/hip_runtime.h>
@@ -26,9 +30,10 @@
 namespace xd {
 
 __global__ void k_batch_reset(uint32_t* counters) {
-    if (threadIdx.x < kQueues) counters[threadIdx.x] = 0;
+    if (threadIdx.x < kClasses * kQueues) counters[threadIdx.x] = 0;
 }
 
+template <int TRAV>
 __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds) {
     const uint32_t sh = V.tw_shift;
     const uint32_t tx = tile % V.tiles_x, ty = tile / V.tiles_x;
@@ -40,40 +45,26 @@ __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, ui
     uint64_t words = 0;
     f3 c;
     float coeff;
-    switch (V.traversal) {
-    case 0: {
+    if constexpr (TRAV == XD_DDA) {
         const float side = max_elem(mk((float)p.dims[0], (float)p.dims[1], (float)p.dims[2]));
         const f3 ro = muls(p.cam.translation, side);
         coeff = emission_coeff(p.view, rd) / side;
         c = p.wide ? dda_trace<false, true>(p, p.macro, ro, rd, words)
                    : dda_trace<false, false>(p, p.macro, ro, rd, words);
-        break;
-    }
-    case 1:
-        c = p.cw ? svo_pixel<1, false, true>(p, rd, lds, nullptr, words)
-                 : svo_pixel<1, false, false>(p, rd, lds, nullptr, words);
+    } else {
+        c = p.cw ? svo_pixel<TRAV, false, true>(p, rd, lds, nullptr, words)
+                 : svo_pixel<TRAV, false, false>(p, rd, lds, nullptr, words);
         coeff = emission_coeff(p.view, rd);
-        break;
-    case 2:
-        c = p.cw ? svo_pixel<2, false, true>(p, rd, lds, nullptr, words)
-                 : svo_pixel<2, false, false>(p, rd, lds, nullptr, words);
-        coeff = emission_coeff(p.view, rd);
-        break;
-    case 3:
-        c = p.cw ? svo_pixel<3, false, true>(p, rd, lds, nullptr, words)
-                 : svo_pixel<3, false, false>(p, rd, lds, nullptr, words);
-        coeff = emission_coeff(p.view, rd);
-        break;
-    default:
-        c = p.cw ? svo_pixel<4, false, true>(p, rd, lds, nullptr, words)
-                 : svo_pixel<4, false, false>(p, rd, lds, nullptr, words);
-        coeff = emission_coeff(p.view, rd);
-        break;
     }
     p.out[(uint64_t)iy * p.out_pitch + ix] = pack_rgba(muls(c, coeff));
 }
 
-__device__ __forceinline__ void batch_body(const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles,
+// The tiles of one traversal class: kQueues ranges (one per XCD) of
+// qbeg/counters.  A wave takes tiles from its own XCD's range with one atomic
+// per grab and moves on to the other ranges when that one is drained; every
+// wave leaves once all ranges are drained.
+template <int TRAV>
+__device__ __forceinline__ void class_body(const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles,
                                            const uint32_t* __restrict__ qbeg, uint32_t* __restrict__ counters,
                                            uint32_t per_grab, uint32_t* lds) {
     const uint32_t lane = threadIdx.x & 63u;
@@ -84,32 +75,45 @@ __device__ __forceinline__ void batch_body(const BatchView* __restrict__ views, 
         for (;;) {
             uint32_t t = 0;
             if (lane == 0) t = atomicAdd(&counters[q], per_grab);
-            t = __shfl(t, 0, 64);
-            if (beg + t >= end) break;  // every wave leaves once each range is drained
+            // wave-uniform from here on: the view's parameters are read with
+            // scalar loads into SGPRs instead of occupying VGPRs
+            t = __builtin_amdgcn_readfirstlane(t);
+            if (beg + t >= end) break;
             const uint32_t stop = min(end, beg + t + per_grab);
             for (uint32_t i = beg + t; i < stop; ++i) {
-                const uint32_t e = tiles[i];
-                shade_tile(views[e >> 20], e & (kMaxTilesPerView - 1u), lane, lds);
+                const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[i]);
+                shade_tile<TRAV>(views[e >> 20], e & (kMaxTilesPerView - 1u), lane, lds);
             }
         }
     }
 }
 
-// Register budget left to the compiler (about 6 waves per SIMD) ...
+// One persistent kernel per traversal class, so each gets the register and
+// LDS budget of its own traversal: the register budget left to the compiler
+// ...
+template <int TRAV>
 __global__ __launch_bounds__(256) void k_batch(const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles,
                                                const uint32_t* __restrict__ qbeg, uint32_t* __restrict__ counters,
                                                uint32_t per_grab) {
     extern __shared__ uint32_t lds[];
-    batch_body(views, tiles, qbeg, counters, per_grab, lds);
+    class_body<TRAV>(views, tiles, qbeg, counters, per_grab, lds);
 }
 
-// ... or capped for 8 waves per SIMD (a few registers spill).
+// ... or capped for 8 waves per SIMD (registers beyond the cap spill).
+template <int TRAV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_w8(
     const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ qbeg,
     uint32_t* __restrict__ counters, uint32_t per_grab) {
     extern __shared__ uint32_t lds[];
-    batch_body(views, tiles, qbeg, counters, per_grab, lds);
+    class_body<TRAV>(views, tiles, qbeg, counters, per_grab, lds);
 }
+
+namespace {
+using BatchKernel = void (*)(const BatchView*, const uint32_t*, const uint32_t*, uint32_t*, uint32_t);
+const BatchKernel kKernels[2][kClasses] = {
+    {k_batch<0>, k_batch<1>, k_batch<2>, k_batch<3>, k_batch<4>},
+    {k_batch_w8<0>, k_batch_w8<1>, k_batch_w8<2>, k_batch_w8<3>, k_batch_w8<4>}};
+}  // namespace
 
 uint32_t tw_shift_of(uint32_t tw) { return tw == 32 ? 5u : (tw == 16 ? 4u : 3u); }
 
@@ -133,16 +137,16 @@ uint32_t batch_view_slots(const BatchView& B) {
 }
 
 size_t batch_lds_bytes(uint32_t slots) {
-    return (size_t)2 * std::max<uint32_t>(slots, 1) * kBlockThreads * sizeof(uint32_t);  // 4 waves x own stacks
+    return slots ? (size_t)2 * slots * kBlockThreads * sizeof(uint32_t) : 0;  // 4 waves x own stacks
 }
 
-int batch_grid(int device, uint32_t slots, bool w8, uint32_t* grid) {
+int batch_grid(int device, int traversal, uint32_t slots, bool w8, uint32_t* grid) {
     hipError_t e;
     int cus = 0, per_cu = 0;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess ||
-        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-             &per_cu, reinterpret_cast<const void*>(w8 ? k_batch_w8 : k_batch), 256, batch_lds_bytes(slots))) !=
-            hipSuccess)
+        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
+                                                          reinterpret_cast<const void*>(kKernels[w8][traversal]),
+                                                          256, batch_lds_bytes(slots))) != hipSuccess)
         return fail_hip(e, "batch occupancy");
     uint32_t g = (uint32_t)std::max(1, cus * std::max(per_cu, 1));
     *grid = (g + kQueues - 1) / kQueues * kQueues;
@@ -150,14 +154,14 @@ int batch_grid(int device, uint32_t slots, bool w8, uint32_t* grid) {
 }
 
 hipError_t launch_batch(const BatchView* views, const uint32_t* tiles, const uint32_t* qbeg, uint32_t* counters,
-                        uint32_t grab, uint32_t slots, bool w8, uint32_t grid, hipStream_t s) {
-    const size_t lds = batch_lds_bytes(slots);
+                        uint32_t grab, const BatchClasses& cls, bool w8, hipStream_t s) {
     grab = std::max<uint32_t>(1, std::min<uint32_t>(grab, 8));
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counters);
-    if (w8)
-        hipLaunchKernelGGL(k_batch_w8, dim3(grid), dim3(256), lds, s, views, tiles, qbeg, counters, grab);
-    else
-        hipLaunchKernelGGL(k_batch, dim3(grid), dim3(256), lds, s, views, tiles, qbeg, counters, grab);
+    for (int c = 0; c < (int)kClasses; ++c) {
+        if (!cls.grid[c]) continue;  // no tiles of this traversal
+        hipLaunchKernelGGL(kKernels[w8][c], dim3(cls.grid[c]), dim3(256), batch_lds_bytes(cls.slots[c]), s, views,
+                           tiles, qbeg + c * kQueues, counters + c * kQueues, grab);
+    }
     return hipGetLastError();
 }
 
@@ -179,8 +183,8 @@ struct xd_batch {
     // captured launch
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    uint32_t graph_slots = 0, graph_grab = 0, graph_w8 = 0;  // configuration the graph was captured with
-    uint32_t grid = 0;
+    BatchClasses graph_cls{};  // per-class grids and LDS stack strides the graph was captured with
+    uint32_t graph_grab = 0, graph_w8 = 0;
 };
 
 namespace {
@@ -213,12 +217,15 @@ int ensure_capacity(xd_batch* b, uint32_t nviews, uint32_t ntiles) {
     return XD_OK;
 }
 
-// Builds and uploads the view and tile tables for the current views/tuning.
-int upload_tables(xd_batch* b, uint32_t* max_slots) {
+// Builds and uploads the view and tile tables for the current views/tuning;
+// *need gets each class's LDS stack stride and, in grid[c], 1 if class c has
+// tiles.  Tiles are grouped by class, and within a class dealt into kQueues
+// ranges by the band of their view they lie in.
+int upload_tables(xd_batch* b, BatchClasses* need) {
     const uint32_t n = (uint32_t)b->views.size();
     std::vector<BatchView> bv(n);
-    std::vector<std::vector<uint32_t>> q(kQueues);
-    uint32_t slots = 1;
+    std::vector<std::vector<uint32_t>> q(kClasses * kQueues);
+    BatchClasses cls{};
     b->total_rays = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const xd_view& v = b->views[i];
@@ -231,23 +238,25 @@ int upload_tables(xd_batch* b, uint32_t* max_slots) {
         if (nt > kMaxTilesPerView)
             return fail(XD_ERR_INVALID_ARGUMENT, "view %u has %llu tiles (limit %u)", i, (unsigned long long)nt,
                         kMaxTilesPerView);
-        slots = std::max(slots, batch_view_slots(B));
+        const uint32_t c = (uint32_t)v.traversal;
+        cls.slots[c] = std::max(cls.slots[c], batch_view_slots(B));
+        cls.grid[c] = 1;
         for (uint32_t ty = 0; ty < tiles_y; ++ty) {
             const uint32_t band = (uint32_t)((uint64_t)ty * kQueues / tiles_y);
-            for (uint32_t tx = 0; tx < B.tiles_x; ++tx) q[band].push_back((i << 20) | (ty * B.tiles_x + tx));
+            for (uint32_t tx = 0; tx < B.tiles_x; ++tx) q[c * kQueues + band].push_back((i << 20) | (ty * B.tiles_x + tx));
         }
         b->total_rays += (uint64_t)W * H;
     }
-    // one LDS layout for the whole launch: every wave's stack region is sized
-    // for the deepest tree of the batch
-    for (uint32_t i = 0; i < n; ++i) bv[i].p.lds_slots = slots;
-    uint32_t qbeg[kQueues + 1];
+    // one LDS layout per class kernel: every wave's stack region is sized for
+    // the deepest tree among that class's views
+    for (uint32_t i = 0; i < n; ++i) bv[i].p.lds_slots = std::max<uint32_t>(cls.slots[bv[i].traversal], 1);
+    uint32_t qbeg[kClasses * kQueues + 1];
     uint32_t total = 0;
-    for (uint32_t k = 0; k < kQueues; ++k) {
+    for (uint32_t k = 0; k < kClasses * kQueues; ++k) {
         qbeg[k] = total;
         total += (uint32_t)q[k].size();
     }
-    qbeg[kQueues] = total;
+    qbeg[kClasses * kQueues] = total;
     int rc = ensure_capacity(b, n, std::max<uint32_t>(total, 1));
     if (rc) return rc;
     std::vector<uint32_t> flat;
@@ -258,40 +267,44 @@ int upload_tables(xd_batch* b, uint32_t* max_slots) {
         (total && (e = hipMemcpy(b->d_tiles, flat.data(), sizeof(uint32_t) * total, hipMemcpyHostToDevice)) != hipSuccess) ||
         (e = hipMemcpy(b->d_qbeg, qbeg, sizeof(qbeg), hipMemcpyHostToDevice)) != hipSuccess)
         return fail_hip(e, "batch tables");
-    *max_slots = slots;
+    *need = cls;
     return XD_OK;
 }
 
-int capture(xd_batch* b, uint32_t slots) {
+int capture(xd_batch* b, const BatchClasses& need) {
     drop_graph(b);
     hipError_t e;
     const bool w8 = b->tuning.waves_per_simd == 8;
-    int rc = batch_grid(b->ctx->device, slots, w8, &b->grid);
-    if (rc) return rc;
+    BatchClasses cls = need;
+    for (int c = 0; c < (int)kClasses; ++c) {
+        if (!need.grid[c]) continue;
+        int rc = batch_grid(b->ctx->device, c, need.slots[c], w8, &cls.grid[c]);
+        if (rc) return rc;
+    }
     hipStream_t s = b->ctx->stream;
     const uint32_t grab = std::max<uint32_t>(1, std::min<uint32_t>(b->tuning.tiles_per_grab, 8));
     if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return fail_hip(e, "hipStreamBeginCapture");
-    const hipError_t le = launch_batch(b->d_views, b->d_tiles, b->d_qbeg, b->d_counters, grab, slots, w8, b->grid, s);
+    const hipError_t le = launch_batch(b->d_views, b->d_tiles, b->d_qbeg, b->d_counters, grab, cls, w8, s);
     if ((e = hipStreamEndCapture(s, &b->graph)) != hipSuccess) return fail_hip(e, "hipStreamEndCapture");
     if (le != hipSuccess) return fail_hip(le, "batch launch capture");
     if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
         return fail_hip(e, "hipGraphInstantiate");
-    b->graph_slots = slots;
+    b->graph_cls = cls;
     b->graph_grab = grab;
-    b->graph_w8 = b->tuning.waves_per_simd == 8;
+    b->graph_w8 = w8;
     return XD_OK;
 }
 
 int prepare(xd_batch* b) {
-    uint32_t slots = 1;
-    int rc = upload_tables(b, &slots);
+    BatchClasses need{};
+    int rc = upload_tables(b, &need);
     if (rc) return rc;
     const uint32_t grab = std::max<uint32_t>(1, std::min<uint32_t>(b->tuning.tiles_per_grab, 8));
-    if (!b->exec || slots > b->graph_slots || grab != b->graph_grab ||
-        b->graph_w8 != (uint32_t)(b->tuning.waves_per_simd == 8))
-        return capture(b, slots);
-    return XD_OK;
+    bool stale = !b->exec || grab != b->graph_grab || b->graph_w8 != (uint32_t)(b->tuning.waves_per_simd == 8);
+    for (int c = 0; c < (int)kClasses; ++c)
+        stale |= (need.grid[c] && !b->graph_cls.grid[c]) || need.slots[c] > b->graph_cls.slots[c];
+    return stale ? capture(b, need) : XD_OK;
 }
 
 int run(xd_batch* b, double* ms) {
@@ -322,8 +335,8 @@ int xd_batch_create(xd_context* ctx, xd_batch** out) {
     b->ctx = ctx;
     for (int t = 0; t < 5; ++t) b->tuning.tile_w[t] = 8;
     b->tuning.tiles_per_grab = 1;
-    if ((e = hipMalloc(&b->d_qbeg, sizeof(uint32_t) * (kQueues + 1))) != hipSuccess ||
-        (e = hipMalloc(&b->d_counters, sizeof(uint32_t) * kQueues)) != hipSuccess) {
+    if ((e = hipMalloc(&b->d_qbeg, sizeof(uint32_t) * (kClasses * kQueues + 1))) != hipSuccess ||
+        (e = hipMalloc(&b->d_counters, sizeof(uint32_t) * kClasses * kQueues)) != hipSuccess) {
         xd_batch_free(b);
         return fail_hip(e, "batch allocation");
     }

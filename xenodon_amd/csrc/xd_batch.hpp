@@ -18,7 +18,8 @@ struct BatchView {
     uint32_t pad;
 };
 
-constexpr uint32_t kQueues = 8;                 // one tile range per XCD
+constexpr uint32_t kQueues = 8;                 // one tile range per XCD, per traversal class
+constexpr uint32_t kClasses = 5;                // one persistent kernel per traversal
 constexpr uint32_t kMaxBatchViews = 1u << 12;   // a tile entry is (view << 20 | tile)
 constexpr uint32_t kMaxTilesPerView = 1u << 20;
 
@@ -33,17 +34,24 @@ void make_batch_view(const xd_volume* vol, int traversal, const xd_camera* cam, 
 // Stack slots a view needs in LDS (0 for traversals without a stack).
 uint32_t batch_view_slots(const BatchView& B);
 
-// Dynamic LDS bytes of a launch whose deepest stack has `slots` entries.
+// Dynamic LDS bytes of a class kernel whose deepest stack has `slots` entries.
 size_t batch_lds_bytes(uint32_t slots);
 
-// Resident blocks of the persistent kernel on the whole device (a multiple
-// of kQueues) for this LDS size and register budget.
-int batch_grid(int device, uint32_t slots, bool w8, uint32_t* grid);
+// Resident blocks of traversal class `traversal`'s persistent kernel on the
+// whole device (a multiple of kQueues) for this LDS size and register budget.
+int batch_grid(int device, int traversal, uint32_t slots, bool w8, uint32_t* grid);
 
-// Enqueues the counter reset and the persistent kernel on s.  views[i].p.lds_slots
-// must equal `slots` for every view; tiles holds qbeg[kQueues] entries split
-// into kQueues ranges.
+// Per-class launch shape: grid[c] = 0 skips class c (no tiles); slots[c] is
+// the LDS stack stride every view of class c carries in p.lds_slots.
+struct BatchClasses {
+    uint32_t grid[kClasses];
+    uint32_t slots[kClasses];
+};
+
+// Enqueues the counter reset and one persistent kernel per present class on
+// s.  qbeg holds kClasses * kQueues + 1 range starts (class c's ranges are
+// entries c*kQueues .. c*kQueues+kQueues); counters holds kClasses * kQueues.
 hipError_t launch_batch(const BatchView* views, const uint32_t* tiles, const uint32_t* qbeg, uint32_t* counters,
-                        uint32_t grab, uint32_t slots, bool w8, uint32_t grid, hipStream_t s);
+                        uint32_t grab, const BatchClasses& cls, bool w8, hipStream_t s);
 
 }  // namespace xd

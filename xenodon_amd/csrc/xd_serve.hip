@@ -149,7 +149,7 @@ struct xd_server {
     uint32_t ring = 2;
     uint64_t step_seq = 0;
     size_t views_off = 0, qbeg_off = 0, counters_off = 0, tiles_off = 0, table_bytes = 0;
-    std::map<uint32_t, uint32_t> grid_of_slots;
+    std::map<uint32_t, uint32_t> grid_of;  // (traversal << 16 | stack slots) -> persistent grid
     uint32_t tile_w[5] = {8, 8, 8, 8, 8};
     uint32_t grab = 1;
     bool w8 = false;
@@ -204,12 +204,13 @@ void admit(xd_server* s, double now) {
     }
 }
 
-uint32_t grid_for(xd_server* s, uint32_t slots) {
-    auto it = s->grid_of_slots.find(slots);
-    if (it != s->grid_of_slots.end()) return it->second;
+uint32_t grid_for(xd_server* s, int traversal, uint32_t slots) {
+    const uint32_t key = (uint32_t)traversal << 16 | slots;
+    auto it = s->grid_of.find(key);
+    if (it != s->grid_of.end()) return it->second;
     uint32_t g = 0;
-    if (batch_grid(s->ctx->device, slots, s->w8, &g) != XD_OK) return 0;
-    s->grid_of_slots[slots] = g;
+    if (batch_grid(s->ctx->device, traversal, slots, s->w8, &g) != XD_OK) return 0;
+    s->grid_of[key] = g;
     return g;
 }
 
@@ -236,30 +237,43 @@ bool launch_step(xd_server* s, Step& S, std::string* err) {
         left -= k;
     }
     const double now = ms_since(s->t0);
-    uint32_t slots = 1;
+    BatchClasses cls{};
     BatchView* views = reinterpret_cast<BatchView*>(S.host + s->views_off);
     uint32_t* tiles = reinterpret_cast<uint32_t*>(S.host + s->tiles_off);
+    // tiles grouped by traversal class (one kernel each), parts in order
+    uint32_t count[kClasses] = {}, at[kClasses];
+    for (size_t i = 0; i < work.size(); ++i) count[work[i]->bv.traversal] += (uint32_t)take[i];
     uint32_t n = 0;
+    for (uint32_t c = 0; c < kClasses; ++c) {
+        at[c] = n;
+        n += count[c];
+    }
     for (size_t i = 0; i < work.size(); ++i) {
         if (!take[i]) continue;
         Request* q = work[i];
-        const uint32_t j = (uint32_t)S.parts.size();
+        const uint32_t j = (uint32_t)S.parts.size(), c = q->bv.traversal;
         S.parts.push_back({q, (uint32_t)q->tiles_sched, (uint32_t)take[i]});
         views[j] = q->bv;
-        slots = std::max(slots, batch_view_slots(q->bv));
-        for (uint64_t t = q->tiles_sched; t < q->tiles_sched + take[i]; ++t) tiles[n++] = (j << 20) | (uint32_t)t;
+        cls.slots[c] = std::max(cls.slots[c], batch_view_slots(q->bv));
+        for (uint64_t t = q->tiles_sched; t < q->tiles_sched + take[i]; ++t) tiles[at[c]++] = (j << 20) | (uint32_t)t;
         q->tiles_sched += take[i];
         if (q->c.launches++ == 0) q->c.start_ms = now;
     }
-    for (size_t j = 0; j < S.parts.size(); ++j) views[j].p.lds_slots = slots;
-    // contiguous runs of the step's tiles, one per XCD
+    for (size_t j = 0; j < S.parts.size(); ++j) views[j].p.lds_slots = std::max<uint32_t>(cls.slots[views[j].traversal], 1);
+    // each class's tiles in contiguous runs, one per XCD
     uint32_t* qbeg = reinterpret_cast<uint32_t*>(S.host + s->qbeg_off);
-    for (uint32_t k = 0; k <= kQueues; ++k) qbeg[k] = (uint32_t)((uint64_t)n * k / kQueues);
-    const uint32_t grid = grid_for(s, slots);
-    if (!grid) {
-        *err = "served step: occupancy query failed";
-        return false;
+    uint32_t base = 0;
+    for (uint32_t c = 0; c < kClasses; ++c) {
+        for (uint32_t k = 0; k < kQueues; ++k) qbeg[c * kQueues + k] = base + (uint32_t)((uint64_t)count[c] * k / kQueues);
+        base += count[c];
+        if (!count[c]) continue;
+        cls.grid[c] = grid_for(s, (int)c, cls.slots[c]);
+        if (!cls.grid[c]) {
+            *err = "served step: occupancy query failed";
+            return false;
+        }
     }
+    qbeg[kClasses * kQueues] = base;
     const size_t bytes = s->tiles_off + (size_t)n * sizeof(uint32_t);
     hipError_t e;
     if ((e = hipMemcpyAsync(S.dev, S.host, bytes, hipMemcpyHostToDevice, S.stream)) != hipSuccess ||
@@ -267,7 +281,7 @@ bool launch_step(xd_server* s, Step& S, std::string* err) {
         (e = launch_batch(reinterpret_cast<const BatchView*>(S.dev + s->views_off),
                           reinterpret_cast<const uint32_t*>(S.dev + s->tiles_off),
                           reinterpret_cast<const uint32_t*>(S.dev + s->qbeg_off),
-                          reinterpret_cast<uint32_t*>(S.dev + s->counters_off), s->grab, slots, s->w8, grid,
+                          reinterpret_cast<uint32_t*>(S.dev + s->counters_off), s->grab, cls, s->w8,
                           S.stream)) != hipSuccess ||
         (e = hipEventRecord(S.ev1, S.stream)) != hipSuccess) {
         *err = std::string("served step launch: ") + hipGetErrorString(e);
@@ -474,8 +488,8 @@ int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** o
     // table layout of one step: views | qbeg | counters | tiles
     s->views_off = 0;
     s->qbeg_off = (sizeof(BatchView) * c.max_active + 255) / 256 * 256;
-    s->counters_off = s->qbeg_off + 64;
-    s->tiles_off = s->counters_off + 64;
+    s->counters_off = s->qbeg_off + 256;  // qbeg: kClasses * kQueues + 1 entries
+    s->tiles_off = s->counters_off + 256;
     s->table_bytes = s->tiles_off + (size_t)c.tiles_per_launch * sizeof(uint32_t);
     s->read_streams.assign(c.readback_threads, nullptr);
     e = hipMalloc(&s->arena_dev, c.frame_arena_bytes);

@@ -148,6 +148,20 @@ __device__ __forceinline__ Planes node_planes(f3 corner, float side, f3 rrd, f3 
     return P;
 }
 
+// t_min / t_max of child box ci of a node: the slab values of node_planes for
+// the child's side of each axis (same operations, same floats).
+__device__ __forceinline__ void child_trange(f3 corner, float side, f3 rrd, f3 bias, uint32_t ci, float& t_min,
+                                             float& t_max) {
+    const float px = (ci & 4u) ? corner.x + side : corner.x;
+    const float py = (ci & 2u) ? corner.y + side : corner.y;
+    const float pz = (ci & 1u) ? corner.z + side : corner.z;
+    const float ax = px * rrd.x - bias.x, bx = (px + side) * rrd.x - bias.x;
+    const float ay = py * rrd.y - bias.y, by = (py + side) * rrd.y - bias.y;
+    const float az = pz * rrd.z - bias.z, bz = (pz + side) * rrd.z - bias.z;
+    t_min = max_elem(mk(gmin(ax, bx), gmin(ay, by), gmin(az, bz)));
+    t_max = min_elem(mk(gmax(ax, bx), gmax(ay, by), gmax(az, bz)));
+}
+
 // Bit k set when child box k of the node is hit (t_min < t_max && t_max > 0,
 // svo_df.comp:25-32).  Per axis the low child spans [T0, T1] and the high
 // child [T1, T2]; min/max of finite floats are exact, so these are the same
@@ -194,8 +208,7 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
     f3 total = splat(0.0f);
     uint4 lo, hi;
     load8(s, node, lo, hi);
-    Planes P = node_planes(corner, side, rrd, bias);
-    uint32_t mask = child_hits(P);
+    uint32_t mask = child_hits(node_planes(corner, side, rrd, bias));
     const uint32_t cap32 = cap > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)cap;
     uint32_t it = 0;
     for (;;) {
@@ -208,11 +221,8 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
             const bool bx = (ci & 4u) != 0, by = (ci & 2u) != 0, bz = (ci & 1u) != 0;
             if (wd & kTagLeaf) {
                 if (!(wd & kTagEmpty)) {  // an empty leaf adds exactly +0
-                    const float mnx = bx ? P.x1 : P.x0, mxx = bx ? P.x2 : P.x1;
-                    const float mny = by ? P.y1 : P.y0, mxy = by ? P.y2 : P.y1;
-                    const float mnz = bz ? P.z1 : P.z0, mxz = bz ? P.z2 : P.z1;
-                    const float t_min = max_elem(mk(gmin(mnx, mxx), gmin(mny, mxy), gmin(mnz, mxz)));
-                    const float t_max = min_elem(mk(gmax(mnx, mxx), gmax(mny, mxy), gmax(mnz, mxz)));
+                    float t_min, t_max;
+                    child_trange(corner, side, rrd, bias, ci, t_min, t_max);
                     total = add(total, muls(unorm_rgb(s.meta[wd & kIndexMask].x), t_max - gmax(t_min, 0.0f)));
                 }
                 if (mask != 0) continue;  // next sibling, from registers
@@ -248,8 +258,7 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
             corner = sub(pos, mk(pmod(pos.x, s2, inv), pmod(pos.y, s2, inv), pmod(pos.z, s2, inv)));
         }
         load8(s, node, lo, hi);
-        P = node_planes(corner, side, rrd, bias);
-        if (!pop) mask = child_hits(P);
+        if (!pop) mask = child_hits(node_planes(corner, side, rrd, bias));
     }
     return total;
 }

@@ -43,6 +43,61 @@ void parallel_z(uint64_t Z, F f) {
 
 inline uint32_t pack(uint32_t r, uint32_t g, uint32_t b, uint32_t a) { return r | (g << 8) | (b << 16) | (a << 24); }
 
+// In-place radix-2 complex FFT of n = 2^k points spaced `stride` apart
+// (inverse when inv; unnormalised).  Twiddles come from a table computed
+// once per size with std::cos/std::sin.
+void fft1d(double* re, double* im, uint64_t n, uint64_t stride, bool inv, const std::vector<double>& cs,
+           const std::vector<double>& sn) {
+    for (uint64_t i = 1, j = 0; i < n; ++i) {
+        uint64_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            std::swap(re[i * stride], re[j * stride]);
+            std::swap(im[i * stride], im[j * stride]);
+        }
+    }
+    for (uint64_t len = 2; len <= n; len <<= 1) {
+        const uint64_t step = n / len;
+        for (uint64_t i = 0; i < n; i += len)
+            for (uint64_t k = 0; k < len / 2; ++k) {
+                const double wr = cs[k * step], wi = inv ? sn[k * step] : -sn[k * step];
+                double* ar = re + (i + k) * stride;
+                double* ai = im + (i + k) * stride;
+                double* br = re + (i + k + len / 2) * stride;
+                double* bi = im + (i + k + len / 2) * stride;
+                const double tr = *br * wr - *bi * wi, ti = *br * wi + *bi * wr;
+                *br = *ar - tr;
+                *bi = *ai - ti;
+                *ar += tr;
+                *ai += ti;
+            }
+    }
+}
+
+// 3-D FFT of an n^3 cube (x fastest) along all three axes.
+void fft3d(std::vector<double>& re, std::vector<double>& im, uint64_t n, bool inv) {
+    std::vector<double> cs(n / 2), sn(n / 2);
+    for (uint64_t k = 0; k < n / 2; ++k) {
+        cs[k] = std::cos(2.0 * M_PI * (double)k / (double)n);
+        sn[k] = std::sin(2.0 * M_PI * (double)k / (double)n);
+    }
+    const uint64_t strides[3] = {1, n, n * n};
+    for (int axis = 0; axis < 3; ++axis) {
+        const uint64_t st = strides[axis];
+        // the n^2 lines along this axis, split over threads by their outer index
+        parallel_z(n, [&](uint64_t a) {
+            for (uint64_t b = 0; b < n; ++b) {
+                uint64_t base;
+                if (axis == 0) base = (a * n + b) * n;
+                else if (axis == 1) base = a * n * n + b;
+                else base = a * n + b;
+                fft1d(re.data() + base, im.data() + base, n, st, inv, cs, sn);
+            }
+        });
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -195,6 +250,120 @@ __attribute__((visibility("default"))) void xds_ct_body(uint8_t* out, uint64_t X
                     uint32_t r = (uint32_t)std::lround(30 + d * (255 - 30));
                     uint32_t g = (uint32_t)std::lround(50 + d * (220 - 50));
                     uint32_t b = (uint32_t)std::lround(160 + d * (150 - 160));
+                    v = pack(r, g, b, 255);
+                }
+                o[(z * Y + y) * X + x] = v;
+            }
+    });
+}
+
+// Cosmological-gas-like volume: a Gaussian random field with power spectrum
+// P(k) ~ k^slope on a base^3 periodic grid (white noise filtered in Fourier
+// space by sqrt(P)), exponentiated into a log-normal density, upsampled
+// trilinearly to X x Y x Z, and thresholded at the `keep_above` quantile of
+// the density (voxels below it are empty, RGBA 0).  Emission is the
+// log-density through a colour ramp (deep red -> orange -> pale yellow),
+// alpha 255.
+__attribute__((visibility("default"))) void xds_lognormal(void* out, uint64_t X, uint64_t Y, uint64_t Z,
+                                                          uint64_t seed, uint32_t base, double slope,
+                                                          double sigma_ln, double keep_above) {
+    const uint64_t n = base, n3 = n * n * n;
+    std::vector<double> re(n3), im(n3, 0.0);
+    Rng rng(seed);
+    for (uint64_t i = 0; i < n3; i += 2) {  // Box-Muller pairs
+        const double u1 = 1.0 - rng.uniform(), u2 = rng.uniform();
+        const double r = std::sqrt(-2.0 * std::log(u1));
+        re[i] = r * std::cos(2.0 * M_PI * u2);
+        if (i + 1 < n3) re[i + 1] = r * std::sin(2.0 * M_PI * u2);
+    }
+    fft3d(re, im, n, false);
+    parallel_z(n, [&](uint64_t z) {
+        const double kz = (double)(z <= n / 2 ? z : n - z);
+        for (uint64_t y = 0; y < n; ++y) {
+            const double ky = (double)(y <= n / 2 ? y : n - y);
+            for (uint64_t x = 0; x < n; ++x) {
+                const double kx = (double)(x <= n / 2 ? x : n - x);
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                const double amp = k2 > 0 ? std::pow(k2, slope / 4.0) : 0.0;  // sqrt(P) = k^(slope/2)
+                const uint64_t i = (z * n + y) * n + x;
+                re[i] *= amp;
+                im[i] *= amp;
+            }
+        }
+    });
+    fft3d(re, im, n, true);
+    // unit variance, then log-normal density
+    double s1 = 0, s2 = 0;
+    for (uint64_t i = 0; i < n3; ++i) {
+        s1 += re[i];
+        s2 += re[i] * re[i];
+    }
+    const double mean = s1 / (double)n3, sd = std::sqrt(std::max(s2 / (double)n3 - mean * mean, 1e-300));
+    std::vector<float> lnrho(n3);
+    for (uint64_t i = 0; i < n3; ++i) lnrho[i] = (float)(sigma_ln * (re[i] - mean) / sd);
+    std::vector<double>().swap(re);
+    std::vector<double>().swap(im);
+    // trilinear upsampling (cell-centred, clamped at the faces)
+    auto sample = [&](uint64_t x, uint64_t y, uint64_t z) {
+        const double fx = std::min(std::max(((double)x + 0.5) * (double)n / (double)X - 0.5, 0.0), (double)(n - 1));
+        const double fy = std::min(std::max(((double)y + 0.5) * (double)n / (double)Y - 0.5, 0.0), (double)(n - 1));
+        const double fz = std::min(std::max(((double)z + 0.5) * (double)n / (double)Z - 0.5, 0.0), (double)(n - 1));
+        const uint64_t x0 = (uint64_t)fx, y0 = (uint64_t)fy, z0 = (uint64_t)fz;
+        const uint64_t x1 = std::min(x0 + 1, n - 1), y1 = std::min(y0 + 1, n - 1), z1 = std::min(z0 + 1, n - 1);
+        const double ax = fx - (double)x0, ay = fy - (double)y0, az = fz - (double)z0;
+        auto at = [&](uint64_t i, uint64_t j, uint64_t k) { return (double)lnrho[(k * n + j) * n + i]; };
+        const double c00 = at(x0, y0, z0) * (1 - ax) + at(x1, y0, z0) * ax;
+        const double c10 = at(x0, y1, z0) * (1 - ax) + at(x1, y1, z0) * ax;
+        const double c01 = at(x0, y0, z1) * (1 - ax) + at(x1, y0, z1) * ax;
+        const double c11 = at(x0, y1, z1) * (1 - ax) + at(x1, y1, z1) * ax;
+        const double c0 = c00 * (1 - ay) + c10 * ay, c1 = c01 * (1 - ay) + c11 * ay;
+        return std::exp(c0 * (1 - az) + c1 * az);
+    };
+    // the threshold: quantile of the upsampled density from a log histogram
+    // of every voxel (exact up to one bin of 1/4096 of the log range)
+    float lo = lnrho[0], hi = lnrho[0];
+    for (float v : lnrho) {
+        lo = std::min(lo, v);
+        hi = std::max(hi, v);
+    }
+    const int bins = 1 << 16;
+    const double blo = (double)lo, bw = ((double)hi - (double)lo) / bins + 1e-12;
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::vector<uint64_t>> hist(nt, std::vector<uint64_t>(bins, 0));
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; ++t)
+            th.emplace_back([&, t] {
+                for (uint64_t z = t; z < Z; z += nt)
+                    for (uint64_t y = 0; y < Y; ++y)
+                        for (uint64_t x = 0; x < X; ++x) {
+                            const int b = (int)std::min<double>(bins - 1, std::max(0.0, (std::log(sample(x, y, z)) - blo) / bw));
+                            ++hist[t][b];
+                        }
+            });
+        for (auto& t : th) t.join();
+    }
+    const uint64_t total = X * Y * Z, want = (uint64_t)(keep_above * (double)total);
+    uint64_t acc = 0;
+    int cut = 0;
+    for (; cut < bins; ++cut) {
+        uint64_t c = 0;
+        for (unsigned t = 0; t < nt; ++t) c += hist[t][cut];
+        if (acc + c > want) break;
+        acc += c;
+    }
+    const double ln_thr = blo + cut * bw, ln_max = (double)hi;
+    uint32_t* o = reinterpret_cast<uint32_t*>(out);
+    parallel_z(Z, [&](uint64_t z) {
+        for (uint64_t y = 0; y < Y; ++y)
+            for (uint64_t x = 0; x < X; ++x) {
+                const double l = std::log(sample(x, y, z));
+                uint32_t v = 0;
+                if (l >= ln_thr) {
+                    const double t = std::min(1.0, (l - ln_thr) / (ln_max - ln_thr + 1e-12));
+                    const uint32_t r = (uint32_t)std::lround(90 + t * (255 - 90));
+                    const uint32_t g = (uint32_t)std::lround(10 + t * (240 - 10));
+                    const uint32_t b = (uint32_t)std::lround(20 + t * t * (170 - 20));
                     v = pack(r, g, b, 255);
                 }
                 o[(z * Y + y) * X + x] = v;

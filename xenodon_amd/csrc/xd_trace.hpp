@@ -90,7 +90,7 @@ __device__ __forceinline__ f3 dda_trace(const RenderParams& p, const uint32_t* _
     bool live = true;
     while (live) {
         constexpr int B = 4;
-        uint64_t addr[B];
+        typename std::conditional<WIDE, uint64_t, uint32_t>::type addr[B];
         float dts[B];
         bool need[B];
 #pragma unroll

@@ -8,6 +8,11 @@ libxd_synth.so, csrc/xd_synth.cpp).
                 random blobs thresholded into a closed thick wall with an
                 interior density gradient; density -> colour by a linear ramp;
                 about 70 % of voxels empty.
+  lognormal()   cosmological-gas-like volume (1024^3 or 2048^3): a Gaussian
+                random field with P(k) ~ k^-3 on a 256^3 grid (in-repo FFT),
+                exponentiated (log-normal), upsampled trilinearly, voxels below
+                the 90th percentile zeroed; emission = log-density through a
+                colour ramp.
 
 Arrays are [Z, Y, X, 4] uint8 (x fastest), the layout of the reference's Grid.
 """
@@ -21,6 +26,7 @@ LIB_PATH = os.path.join(PKG, "libxd_synth.so")
 
 SEED_SHELLS = 0x5EED0001C0FFEE01
 SEED_CT = 0x5EED0002C0FFEE02
+SEED_LOGNORMAL = 0x5EED0003C0FFEE03
 
 _lib = None
 
@@ -35,6 +41,8 @@ def lib():
         L.xds_hash.restype = ctypes.c_uint64
         L.xds_shells.argtypes = [ctypes.c_void_p] + [ctypes.c_uint64] * 4 + [ctypes.c_uint32] + [ctypes.c_double] * 5
         L.xds_ct_body.argtypes = [ctypes.c_void_p] + [ctypes.c_uint64] * 4 + [ctypes.c_uint32] + [ctypes.c_double] * 3
+        L.xds_lognormal.argtypes = [ctypes.c_void_p] + [ctypes.c_uint64] * 4 + [ctypes.c_uint32] + \
+            [ctypes.c_double] * 3
         _lib = L
     return _lib
 
@@ -61,4 +69,11 @@ def ct_body(shape_xyz=(512, 361, 512), seed=SEED_CT, n_blobs=40, sigma=6.0, empt
     X, Y, Z = shape_xyz
     out = np.empty((Z, Y, X, 4), np.uint8)
     lib().xds_ct_body(out.ctypes.data, X, Y, Z, seed, n_blobs, sigma, empty, wall)
+    return out
+
+
+def lognormal(side=1024, seed=SEED_LOGNORMAL, base=256, slope=-3.0, sigma_ln=1.0, keep_above=0.90):
+    """side^3 log-normal volume ([Z, Y, X, 4] uint8); 1024 -> 4 GiB, 2048 -> 32 GiB."""
+    out = np.empty((side, side, side, 4), np.uint8)
+    lib().xds_lognormal(out.ctypes.data, side, side, side, seed, base, slope, sigma_ln, keep_above)
     return out
