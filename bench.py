@@ -320,7 +320,7 @@ def main():
     for name in (travs if args.per_traversal_groups > 0 else []):
         vol, md = vols[name]
         sub = ctx.batch()
-        sub.set_tuning(tuning["tile_w"], tuning["tiles_per_grab"], tuning["waves_per_simd"])
+        sub.set_tuning(tuning["tile_w"], tuning["tiles_per_grab"], tuning["waves_per_simd"], tuning["class_kernels"])
         ms_sum = 0.0
         rays_sum = 0
         n_groups = min(len(groups), args.per_traversal_groups)
@@ -360,12 +360,14 @@ def main():
         svols = {"ct512_grid": ("grid", grid, [512, 361, 512]), "ct512_svo": ("svo", (nodes, side), [side] * 3),
                  "shells128_grid": ("grid", shells, [128] * 3),
                  "shells128_svo": ("svo", (snodes, sst["side"]), [sst["side"]] * 3)}
+        # served steps are small: one kernel for every traversal per step
+        serve_tuning = dict(tuning, class_kernels=0)
         for k, r in enumerate(rates):
-            res = served_leg(ctx, svols, r, args.serve_seconds, seed=1000 + k, tuning=tuning)
+            res = served_leg(ctx, svols, r, args.serve_seconds, seed=1000 + k, tuning=serve_tuning)
             log(f"served @ {r:.0f} req/s: {res['mrays_s']} Mrays/s, p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms")
             served.append(res)
         if args.serve_saturated:
-            res = served_leg(ctx, svols, 0, 0, seed=999, tuning=tuning, n_saturated=args.serve_saturated)
+            res = served_leg(ctx, svols, 0, 0, seed=999, tuning=serve_tuning, n_saturated=args.serve_saturated)
             log(f"served, {args.serve_saturated} requests at once: {res['mrays_s']} Mrays/s, p99 {res['p99_ms']} ms")
             served.append(res)
 

@@ -186,12 +186,16 @@ typedef struct {
 
 /* Launch shape of a batch: per traversal the wavefront tile (8 = 8x8,
  * 16 = 16x4, 32 = 32x2 pixels), the tiles a wavefront takes per queue grab
- * (1..8), and the register budget of the persistent kernel (0 = compiler's
- * choice, 8 = capped for 8 waves per SIMD).  Results never depend on it. */
+ * (1..8), the register budget of the persistent kernels (0 = compiler's
+ * choice, 8 = capped for 8 waves per SIMD), and whether one persistent kernel
+ * drains every traversal's tiles (class_kernels = 0) or each traversal gets
+ * its own kernel, with registers and LDS of that traversal only (1).
+ * Results never depend on it. */
 typedef struct {
     uint32_t tile_w[5];
     uint32_t tiles_per_grab;
     uint32_t waves_per_simd;
+    uint32_t class_kernels;
 } xd_batch_tuning;
 
 typedef struct xd_batch xd_batch;

@@ -65,11 +65,11 @@ def test_batch_matches_plain_renders(env):
     _check_against_plain(ctx, views)
 
 
-@pytest.mark.parametrize("tile_w,grab,waves", [(16, 1, 0), (32, 2, 8), (8, 4, 8)])
-def test_batch_launch_shapes_do_not_change_pixels(env, tile_w, grab, waves):
+@pytest.mark.parametrize("tile_w,grab,waves,classes", [(16, 1, 0, 0), (32, 2, 8, 1), (8, 4, 8, 0), (8, 1, 0, 1)])
+def test_batch_launch_shapes_do_not_change_pixels(env, tile_w, grab, waves, classes):
     ctx, vols = env
     b = ctx.batch()
-    b.set_tuning([tile_w, 8, tile_w, 32, 16], grab, waves)
+    b.set_tuning([tile_w, 8, tile_w, 32, 16], grab, waves, classes)
     views = _views(ctx, vols, seed=2)
     b.set_views(views)
     b.render()

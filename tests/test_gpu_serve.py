@@ -73,9 +73,10 @@ def _drain(srv, count):
     return got
 
 
-def test_served_frames_match_plain(env):
+@pytest.mark.parametrize("class_kernels", [0, 1])
+def test_served_frames_match_plain(env, class_kernels):
     ctx, data, plain = env
-    srv = ctx.server(tiles_per_launch=1024)
+    srv = ctx.server(tiles_per_launch=1024, tuning={"tile_w": [8, 16, 8, 8, 32], "class_kernels": class_kernels})
     ids = _load_all(srv, data)
     reqs = _random_requests(40, seed=5, data=data)
     tags = {}

@@ -69,7 +69,7 @@ class View(ctypes.Structure):
 
 class BatchTuning(ctypes.Structure):
     _fields_ = [("tile_w", ctypes.c_uint32 * 5), ("tiles_per_grab", ctypes.c_uint32),
-                ("waves_per_simd", ctypes.c_uint32)]
+                ("waves_per_simd", ctypes.c_uint32), ("class_kernels", ctypes.c_uint32)]
 
 
 class BuildStats(ctypes.Structure):
@@ -275,6 +275,7 @@ class Context:
             c.tuning.tile_w[:] = list(tuning.get("tile_w", [8] * 5))
             c.tuning.tiles_per_grab = tuning.get("tiles_per_grab", 1)
             c.tuning.waves_per_simd = tuning.get("waves_per_simd", 0)
+            c.tuning.class_kernels = tuning.get("class_kernels", 0)
         h = ctypes.c_void_p()
         check(lib().xd_server_create(self.h, ctypes.byref(c), ctypes.byref(h)))
         return self._adopt(Server(self, h))
@@ -336,19 +337,22 @@ class Batch:
     def tuning(self):
         t = BatchTuning()
         check(lib().xd_batch_get_tuning(self.h, ctypes.byref(t)))
-        return {"tile_w": list(t.tile_w), "tiles_per_grab": t.tiles_per_grab, "waves_per_simd": t.waves_per_simd}
+        return {"tile_w": list(t.tile_w), "tiles_per_grab": t.tiles_per_grab, "waves_per_simd": t.waves_per_simd,
+                "class_kernels": t.class_kernels}
 
-    def set_tuning(self, tile_w, tiles_per_grab=1, waves_per_simd=0):
+    def set_tuning(self, tile_w, tiles_per_grab=1, waves_per_simd=0, class_kernels=0):
         t = BatchTuning()
         t.tile_w[:] = list(tile_w)
         t.tiles_per_grab = tiles_per_grab
         t.waves_per_simd = waves_per_simd
+        t.class_kernels = class_kernels
         check(lib().xd_batch_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, trials=3):
         t = BatchTuning()
         check(lib().xd_batch_autotune(self.h, trials, ctypes.byref(t)))
-        return {"tile_w": list(t.tile_w), "tiles_per_grab": t.tiles_per_grab, "waves_per_simd": t.waves_per_simd}
+        return {"tile_w": list(t.tile_w), "tiles_per_grab": t.tiles_per_grab, "waves_per_simd": t.waves_per_simd,
+                "class_kernels": t.class_kernels}
 
     def render(self):
         """Returns (ms, rays)."""

@@ -34,6 +34,21 @@ __global__ void k_batch_reset(uint32_t* counters) {
 }
 
 template <int TRAV>
+__device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds);
+
+// Any traversal: the kernel for all classes at once dispatches on the view.
+template <>
+__device__ __forceinline__ void shade_tile<kClasses>(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds) {
+    switch (V.traversal) {
+    case XD_DDA: shade_tile<XD_DDA>(V, tile, lane, lds); break;
+    case XD_SVO_NAIVE: shade_tile<XD_SVO_NAIVE>(V, tile, lane, lds); break;
+    case XD_ESVO: shade_tile<XD_ESVO>(V, tile, lane, lds); break;
+    case XD_SVO_DF: shade_tile<XD_SVO_DF>(V, tile, lane, lds); break;
+    default: shade_tile<XD_SVO_ROPE>(V, tile, lane, lds); break;
+    }
+}
+
+template <int TRAV>
 __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds) {
     const uint32_t sh = V.tw_shift;
     const uint32_t tx = tile % V.tiles_x, ty = tile / V.tiles_x;
@@ -69,12 +84,16 @@ __device__ __forceinline__ void class_body(const BatchView* __restrict__ views, 
                                            uint32_t per_grab, uint32_t* lds) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t home = blockIdx.x % kQueues;
-    for (uint32_t qi = 0; qi < kQueues; ++qi) {
-        const uint32_t q = (home + qi) % kQueues;
-        const uint32_t beg = qbeg[q], end = qbeg[q + 1];
+    constexpr uint32_t kRanges = TRAV == (int)kClasses ? kClasses * kQueues : kQueues;
+    for (uint32_t qi = 0; qi < kRanges; ++qi) {
+        // this XCD's range first (of every class, for the all-class kernel),
+        // then the next XCD's
+        const uint32_t qq = TRAV == (int)kClasses ? (qi % kClasses) * kQueues + (home + qi / kClasses) % kQueues
+                                                  : (home + qi) % kQueues;
+        const uint32_t beg = qbeg[qq], end = qbeg[qq + 1];
         for (;;) {
             uint32_t t = 0;
-            if (lane == 0) t = atomicAdd(&counters[q], per_grab);
+            if (lane == 0) t = atomicAdd(&counters[qq], per_grab);
             // wave-uniform from here on: the view's parameters are read with
             // scalar loads into SGPRs instead of occupying VGPRs
             t = __builtin_amdgcn_readfirstlane(t);
@@ -110,9 +129,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
 
 namespace {
 using BatchKernel = void (*)(const BatchView*, const uint32_t*, const uint32_t*, uint32_t*, uint32_t);
-const BatchKernel kKernels[2][kClasses] = {
-    {k_batch<0>, k_batch<1>, k_batch<2>, k_batch<3>, k_batch<4>},
-    {k_batch_w8<0>, k_batch_w8<1>, k_batch_w8<2>, k_batch_w8<3>, k_batch_w8<4>}};
+const BatchKernel kKernels[2][kClasses + 1] = {
+    {k_batch<0>, k_batch<1>, k_batch<2>, k_batch<3>, k_batch<4>, k_batch<kClasses>},
+    {k_batch_w8<0>, k_batch_w8<1>, k_batch_w8<2>, k_batch_w8<3>, k_batch_w8<4>, k_batch_w8<kClasses>}};
 }  // namespace
 
 uint32_t tw_shift_of(uint32_t tw) { return tw == 32 ? 5u : (tw == 16 ? 4u : 3u); }
@@ -157,10 +176,11 @@ hipError_t launch_batch(const BatchView* views, const uint32_t* tiles, const uin
                         uint32_t grab, const BatchClasses& cls, bool w8, hipStream_t s) {
     grab = std::max<uint32_t>(1, std::min<uint32_t>(grab, 8));
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counters);
-    for (int c = 0; c < (int)kClasses; ++c) {
-        if (!cls.grid[c]) continue;  // no tiles of this traversal
+    for (int c = 0; c <= (int)kClasses; ++c) {
+        if (!cls.grid[c]) continue;  // no tiles for this kernel
+        const uint32_t base = c < (int)kClasses ? c * kQueues : 0;
         hipLaunchKernelGGL(kKernels[w8][c], dim3(cls.grid[c]), dim3(256), batch_lds_bytes(cls.slots[c]), s, views,
-                           tiles, qbeg + c * kQueues, counters + c * kQueues, grab);
+                           tiles, qbeg + base, counters + base, grab);
     }
     return hipGetLastError();
 }
@@ -247,9 +267,20 @@ int upload_tables(xd_batch* b, BatchClasses* need) {
         }
         b->total_rays += (uint64_t)W * H;
     }
-    // one LDS layout per class kernel: every wave's stack region is sized for
-    // the deepest tree among that class's views
-    for (uint32_t i = 0; i < n; ++i) bv[i].p.lds_slots = std::max<uint32_t>(cls.slots[bv[i].traversal], 1);
+    // one LDS layout per kernel: every wave's stack region is sized for the
+    // deepest tree among the views that kernel drains
+    if (!b->tuning.class_kernels) {
+        uint32_t all = 0, any = 0;
+        for (uint32_t c = 0; c < kClasses; ++c) {
+            all = std::max(all, cls.slots[c]);
+            any |= cls.grid[c];
+            cls.grid[c] = cls.slots[c] = 0;
+        }
+        cls.grid[kClasses] = any;
+        cls.slots[kClasses] = all;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        bv[i].p.lds_slots = std::max<uint32_t>(cls.slots[b->tuning.class_kernels ? bv[i].traversal : kClasses], 1);
     uint32_t qbeg[kClasses * kQueues + 1];
     uint32_t total = 0;
     for (uint32_t k = 0; k < kClasses * kQueues; ++k) {
@@ -276,7 +307,7 @@ int capture(xd_batch* b, const BatchClasses& need) {
     hipError_t e;
     const bool w8 = b->tuning.waves_per_simd == 8;
     BatchClasses cls = need;
-    for (int c = 0; c < (int)kClasses; ++c) {
+    for (int c = 0; c <= (int)kClasses; ++c) {
         if (!need.grid[c]) continue;
         int rc = batch_grid(b->ctx->device, c, need.slots[c], w8, &cls.grid[c]);
         if (rc) return rc;
@@ -302,7 +333,7 @@ int prepare(xd_batch* b) {
     if (rc) return rc;
     const uint32_t grab = std::max<uint32_t>(1, std::min<uint32_t>(b->tuning.tiles_per_grab, 8));
     bool stale = !b->exec || grab != b->graph_grab || b->graph_w8 != (uint32_t)(b->tuning.waves_per_simd == 8);
-    for (int c = 0; c < (int)kClasses; ++c)
+    for (int c = 0; c <= (int)kClasses; ++c)
         stale |= (need.grid[c] && !b->graph_cls.grid[c]) || need.slots[c] > b->graph_cls.slots[c];
     return stale ? capture(b, need) : XD_OK;
 }
@@ -375,6 +406,7 @@ int xd_batch_set_tuning(xd_batch* b, const xd_batch_tuning* t) {
         return fail(XD_ERR_INVALID_ARGUMENT, "tiles_per_grab must be 1..8");
     if (t->waves_per_simd != 0 && t->waves_per_simd != 8)
         return fail(XD_ERR_INVALID_ARGUMENT, "waves_per_simd must be 0 (compiler's choice) or 8");
+    if (t->class_kernels > 1) return fail(XD_ERR_INVALID_ARGUMENT, "class_kernels must be 0 or 1");
     b->tuning = *t;
     if (b->views.empty()) return XD_OK;
     hipError_t e = hipSetDevice(b->ctx->device);
@@ -451,6 +483,24 @@ int xd_batch_autotune(xd_batch* b, uint32_t trials, xd_batch_tuning* chosen) {
         }
     }
     b->tuning.tiles_per_grab = best.tiles_per_grab;
+    // one kernel for every traversal, or one per traversal
+    best_ms = 1e30;
+    for (uint32_t k : {0u, 1u}) {
+        b->tuning.class_kernels = k;
+        int rc = prepare(b);
+        if (rc) return rc;
+        double m = 1e30;
+        for (uint32_t r = 0; r < trials; ++r) {
+            double ms = 0;
+            if ((rc = run(b, &ms))) return rc;
+            m = std::min(m, ms);
+        }
+        if (m < best_ms) {
+            best_ms = m;
+            best.class_kernels = k;
+        }
+    }
+    b->tuning.class_kernels = best.class_kernels;
     // register budget: compiler's choice or 8 waves per SIMD
     best_ms = 1e30;
     for (uint32_t w : {0u, 8u}) {

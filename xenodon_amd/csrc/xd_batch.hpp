@@ -37,20 +37,23 @@ uint32_t batch_view_slots(const BatchView& B);
 // Dynamic LDS bytes of a class kernel whose deepest stack has `slots` entries.
 size_t batch_lds_bytes(uint32_t slots);
 
-// Resident blocks of traversal class `traversal`'s persistent kernel on the
-// whole device (a multiple of kQueues) for this LDS size and register budget.
+// Resident blocks of traversal class `traversal`'s persistent kernel (or, for
+// traversal == kClasses, of the all-class kernel) on the whole device (a
+// multiple of kQueues) for this LDS size and register budget.
 int batch_grid(int device, int traversal, uint32_t slots, bool w8, uint32_t* grid);
 
-// Per-class launch shape: grid[c] = 0 skips class c (no tiles); slots[c] is
-// the LDS stack stride every view of class c carries in p.lds_slots.
+// Launch shape: entry c < kClasses is traversal c's own kernel, entry
+// kClasses the kernel that drains every class's tiles.  grid[c] = 0 skips
+// entry c; slots[c] is the LDS stack stride that every view drained by that
+// kernel carries in p.lds_slots.
 struct BatchClasses {
-    uint32_t grid[kClasses];
-    uint32_t slots[kClasses];
+    uint32_t grid[kClasses + 1];
+    uint32_t slots[kClasses + 1];
 };
 
-// Enqueues the counter reset and one persistent kernel per present class on
-// s.  qbeg holds kClasses * kQueues + 1 range starts (class c's ranges are
-// entries c*kQueues .. c*kQueues+kQueues); counters holds kClasses * kQueues.
+// Enqueues the counter reset and the present kernels of cls on s.  qbeg holds
+// kClasses * kQueues + 1 range starts (class c's ranges are entries
+// c*kQueues .. c*kQueues+kQueues); counters holds kClasses * kQueues.
 hipError_t launch_batch(const BatchView* views, const uint32_t* tiles, const uint32_t* qbeg, uint32_t* counters,
                         uint32_t grab, const BatchClasses& cls, bool w8, hipStream_t s);
 

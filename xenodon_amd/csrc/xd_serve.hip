@@ -153,6 +153,7 @@ struct xd_server {
     uint32_t tile_w[5] = {8, 8, 8, 8, 8};
     uint32_t grab = 1;
     bool w8 = false;
+    bool class_kernels = false;
 
     // state
     bool stop = false, reader_stop = false;
@@ -241,7 +242,7 @@ bool launch_step(xd_server* s, Step& S, std::string* err) {
     BatchView* views = reinterpret_cast<BatchView*>(S.host + s->views_off);
     uint32_t* tiles = reinterpret_cast<uint32_t*>(S.host + s->tiles_off);
     // tiles grouped by traversal class (one kernel each), parts in order
-    uint32_t count[kClasses] = {}, at[kClasses];
+    uint32_t count[kClasses] = {}, at[kClasses];  // tiles per class, in class order
     for (size_t i = 0; i < work.size(); ++i) count[work[i]->bv.traversal] += (uint32_t)take[i];
     uint32_t n = 0;
     for (uint32_t c = 0; c < kClasses; ++c) {
@@ -259,20 +260,29 @@ bool launch_step(xd_server* s, Step& S, std::string* err) {
         q->tiles_sched += take[i];
         if (q->c.launches++ == 0) q->c.start_ms = now;
     }
-    for (size_t j = 0; j < S.parts.size(); ++j) views[j].p.lds_slots = std::max<uint32_t>(cls.slots[views[j].traversal], 1);
     // each class's tiles in contiguous runs, one per XCD
     uint32_t* qbeg = reinterpret_cast<uint32_t*>(S.host + s->qbeg_off);
-    uint32_t base = 0;
+    uint32_t base = 0, all_slots = 0;
     for (uint32_t c = 0; c < kClasses; ++c) {
         for (uint32_t k = 0; k < kQueues; ++k) qbeg[c * kQueues + k] = base + (uint32_t)((uint64_t)count[c] * k / kQueues);
         base += count[c];
-        if (!count[c]) continue;
+        all_slots = std::max(all_slots, cls.slots[c]);
+    }
+    if (!s->class_kernels) {  // one kernel drains every class
+        for (uint32_t c = 0; c < kClasses; ++c) cls.slots[c] = 0;
+        cls.slots[kClasses] = all_slots;
+    }
+    for (uint32_t c = 0; c <= kClasses; ++c) {
+        const bool present = s->class_kernels ? (c < kClasses && count[c]) : c == kClasses;
+        if (!present) continue;
         cls.grid[c] = grid_for(s, (int)c, cls.slots[c]);
         if (!cls.grid[c]) {
             *err = "served step: occupancy query failed";
             return false;
         }
     }
+    for (size_t j = 0; j < S.parts.size(); ++j)
+        views[j].p.lds_slots = std::max<uint32_t>(cls.slots[s->class_kernels ? views[j].traversal : kClasses], 1);
     qbeg[kClasses * kQueues] = base;
     const size_t bytes = s->tiles_off + (size_t)n * sizeof(uint32_t);
     hipError_t e;
@@ -476,6 +486,7 @@ int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** o
     if (c.tuning.tiles_per_grab > 8) return fail(XD_ERR_INVALID_ARGUMENT, "tiles_per_grab must be 1..8");
     if (c.tuning.waves_per_simd != 0 && c.tuning.waves_per_simd != 8)
         return fail(XD_ERR_INVALID_ARGUMENT, "waves_per_simd must be 0 or 8");
+    if (c.tuning.class_kernels > 1) return fail(XD_ERR_INVALID_ARGUMENT, "class_kernels must be 0 or 1");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
     auto* s = new xd_server();
@@ -484,6 +495,7 @@ int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** o
     for (int t = 0; t < 5; ++t) s->tile_w[t] = c.tuning.tile_w[t];
     s->grab = c.tuning.tiles_per_grab;
     s->w8 = c.tuning.waves_per_simd == 8;
+    s->class_kernels = c.tuning.class_kernels != 0;
     s->ring = c.launches_in_flight;
     // table layout of one step: views | qbeg | counters | tiles
     s->views_off = 0;

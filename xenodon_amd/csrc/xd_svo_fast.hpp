@@ -185,8 +185,8 @@ __device__ __forceinline__ uint32_t child_hits(const Planes& P) {
 // eight box tests are done at once into a hit mask, and only hit children are
 // visited -- still in index order, so leaves are summed in the shader's order
 // with the shader's t values.  A stack entry is (node, children still to
-// visit, depth); a node with none left is not pushed (the shader pops it and
-// finds nothing more to do).  Each iteration visits one hit child and then
+// visit, last visited child, depth); a node with none left is not pushed (the
+// shader pops it and finds nothing more to do).  Each iteration visits one hit child and then
 // enters at most one node -- the child (descent) or the nearest ancestor with
 // children left (pop) -- through one shared path, so lanes that descend and
 // lanes that pop wait on the same loads.  The child position the shader
@@ -195,7 +195,10 @@ __device__ __forceinline__ uint32_t child_hits(const Planes& P) {
 // svo_df.comp:50-61, from a point inside the node being finished, which lies
 // inside every ancestor).  The popped node's depth is the depth field the
 // shader reads (svo_df.comp:58); when every depth field equals the node's
-// depth in the tree, the depth kept on the stack is that same value.
+// depth in the tree, the depth kept on the stack is that same value.  When
+// they do not, a popped node can be placed elsewhere than where it was
+// entered, so the walk follows the shader literally: every non-last child
+// pushes, and a popped node's remaining children are tested again.
 __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t slots, f3 ro, f3 rd,
                                       uint32_t* stk_node, uint32_t* stk_child) {
     f3 rrd = divv(splat(1.0f), rd);
@@ -217,6 +220,7 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
             if (++it > cap32) break;
             const uint32_t ci = (uint32_t)__builtin_ctz(mask);
             mask &= mask - 1u;
+            bool push;
             const uint32_t wd = pick8(lo, hi, ci);
             const bool bx = (ci & 4u) != 0, by = (ci & 2u) != 0, bz = (ci & 1u) != 0;
             if (wd & kTagLeaf) {
@@ -226,10 +230,10 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
                     total = add(total, muls(unorm_rgb(s.meta[wd & kIndexMask].x), t_max - gmax(t_min, 0.0f)));
                 }
                 if (mask != 0) continue;  // next sibling, from registers
-            } else if (mask == 0 || sp < cap_sp) {
-                if (mask != 0) {
+            } else if (!(push = s.depth_exact ? mask != 0 : ci != 7u) || sp < cap_sp) {
+                if (push) {
                     stk_node[sp * kWaveLanes] = node;
-                    stk_child[sp * kWaveLanes] = mask | (depth << 8);
+                    stk_child[sp * kWaveLanes] = mask | (ci << 8) | (depth << 11);
                     ++sp;
                 }
                 corner = mk(bx ? corner.x + side : corner.x, by ? corner.y + side : corner.y,
@@ -248,7 +252,7 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
             node = min(stk_node[sp * kWaveLanes], s.n - 1u);
             const uint32_t e = stk_child[sp * kWaveLanes];
             mask = e & 0xFFu;
-            depth = (e >> 8) & 0xFFu;
+            depth = (e >> 11) & 0xFFu;
             const uint32_t d = s.depth_exact ? depth : (s.meta[node].y & kDepthMask);
             const f3 pos = mk(corner.x + side, corner.y + side, corner.z + side);
             side = __builtin_ldexpf(1.0f, -(int)d) * 0.5f;
@@ -258,7 +262,15 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
             corner = sub(pos, mk(pmod(pos.x, s2, inv), pmod(pos.y, s2, inv), pmod(pos.z, s2, inv)));
         }
         load8(s, node, lo, hi);
-        if (!pop) mask = child_hits(node_planes(corner, side, rrd, bias));
+        if (!pop) {
+            mask = child_hits(node_planes(corner, side, rrd, bias));
+        } else if (!s.depth_exact) {
+            // the shader tests the remaining children at the geometry the
+            // depth field gives the popped node, which may differ from the
+            // geometry they were first tested at
+            const uint32_t ci = (stk_child[sp * kWaveLanes] >> 8) & 7u;
+            mask = child_hits(node_planes(corner, side, rrd, bias)) & (0xFEu << ci) & 0xFFu;
+        }
     }
     return total;
 }
