@@ -320,7 +320,8 @@ def main():
     for name in (travs if args.per_traversal_groups > 0 else []):
         vol, md = vols[name]
         sub = ctx.batch()
-        sub.set_tuning(tuning["tile_w"], tuning["tiles_per_grab"], tuning["waves_per_simd"], tuning["class_kernels"])
+        sub.set_tuning(tuning["tile_w"], tuning["tiles_per_grab"], tuning["waves_per_simd"], tuning["class_kernels"],
+                       tuning["interleave"])
         ms_sum = 0.0
         rays_sum = 0
         n_groups = min(len(groups), args.per_traversal_groups)

@@ -189,13 +189,17 @@ typedef struct {
  * (1..8), the register budget of the persistent kernels (0 = compiler's
  * choice, 8 = capped for 8 waves per SIMD), and whether one persistent kernel
  * drains every traversal's tiles (class_kernels = 0) or each traversal gets
- * its own kernel, with registers and LDS of that traversal only (1).
+ * its own kernel, with registers and LDS of that traversal only (1), and the
+ * tile order: 0 = view by view, 1 = views of the same volume, traversal and
+ * size interleaved tile by tile, so that the same screen tile of, e.g.,
+ * consecutive camera frames is traced at the same time through the same L2.
  * Results never depend on it. */
 typedef struct {
     uint32_t tile_w[5];
     uint32_t tiles_per_grab;
     uint32_t waves_per_simd;
     uint32_t class_kernels;
+    uint32_t interleave;
 } xd_batch_tuning;
 
 typedef struct xd_batch xd_batch;

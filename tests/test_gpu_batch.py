@@ -65,11 +65,12 @@ def test_batch_matches_plain_renders(env):
     _check_against_plain(ctx, views)
 
 
-@pytest.mark.parametrize("tile_w,grab,waves,classes", [(16, 1, 0, 0), (32, 2, 8, 1), (8, 4, 8, 0), (8, 1, 0, 1)])
-def test_batch_launch_shapes_do_not_change_pixels(env, tile_w, grab, waves, classes):
+@pytest.mark.parametrize("tile_w,grab,waves,classes,inter", [(16, 1, 0, 0, 1), (32, 2, 8, 1, 0), (8, 4, 8, 0, 0),
+                                                             (8, 1, 0, 1, 1)])
+def test_batch_launch_shapes_do_not_change_pixels(env, tile_w, grab, waves, classes, inter):
     ctx, vols = env
     b = ctx.batch()
-    b.set_tuning([tile_w, 8, tile_w, 32, 16], grab, waves, classes)
+    b.set_tuning([tile_w, 8, tile_w, 32, 16], grab, waves, classes, inter)
     views = _views(ctx, vols, seed=2)
     b.set_views(views)
     b.render()
@@ -97,6 +98,7 @@ def test_batch_matches_oracle_on_c1(env):
     ov = xo.OracleVolume(grid=grid)
     vol, md = vols["shells_grid"]
     b = ctx.batch()
+    b.set_tuning([8] * 5, 1, 0, 0, 1)  # the four views form one interleaved group
     cams = H.orbit_cameras(4, 2 * np.sqrt(3), seed=7)
     views = [(vol, abi.DDA, abi.make_camera(*c), abi.make_uniforms(256, 256, md, emission=4.0), ctx.frame(256, 256))
              for c in cams]

@@ -245,6 +245,12 @@ int upload_tables(xd_batch* b, BatchClasses* need) {
     const uint32_t n = (uint32_t)b->views.size();
     std::vector<BatchView> bv(n);
     std::vector<std::vector<uint32_t>> q(kClasses * kQueues);
+    struct Group {
+        uint64_t key[6];
+        std::vector<uint32_t> views;
+        uint32_t tiles_x, tiles_y;
+    };
+    std::vector<Group> groups;
     BatchClasses cls{};
     b->total_rays = 0;
     for (uint32_t i = 0; i < n; ++i) {
@@ -261,12 +267,31 @@ int upload_tables(xd_batch* b, BatchClasses* need) {
         const uint32_t c = (uint32_t)v.traversal;
         cls.slots[c] = std::max(cls.slots[c], batch_view_slots(B));
         cls.grid[c] = 1;
-        for (uint32_t ty = 0; ty < tiles_y; ++ty) {
-            const uint32_t band = (uint32_t)((uint64_t)ty * kQueues / tiles_y);
-            for (uint32_t tx = 0; tx < B.tiles_x; ++tx) q[c * kQueues + band].push_back((i << 20) | (ty * B.tiles_x + tx));
+        if (b->tuning.interleave) {
+            // views of one volume, traversal, size and tile shape form a group
+            // whose tiles are dealt together below
+            const uint64_t key[6] = {(uint64_t)(uintptr_t)v.volume, (uint64_t)c, W, H, B.tw_shift, 0};
+            size_t g = 0;
+            while (g < groups.size() && memcmp(groups[g].key, key, sizeof key) != 0) ++g;
+            if (g == groups.size()) groups.push_back({{key[0], key[1], key[2], key[3], key[4], key[5]}, {}, B.tiles_x, tiles_y});
+            groups[g].views.push_back(i);
+        } else {
+            for (uint32_t ty = 0; ty < tiles_y; ++ty) {
+                const uint32_t band = (uint32_t)((uint64_t)ty * kQueues / tiles_y);
+                for (uint32_t tx = 0; tx < B.tiles_x; ++tx)
+                    q[c * kQueues + band].push_back((i << 20) | (ty * B.tiles_x + tx));
+            }
         }
         b->total_rays += (uint64_t)W * H;
     }
+    // interleaved: the same tile of every view of a group, one after another
+    for (const auto& G : groups)
+        for (uint32_t ty = 0; ty < G.tiles_y; ++ty) {
+            const uint32_t band = (uint32_t)((uint64_t)ty * kQueues / G.tiles_y);
+            for (uint32_t tx = 0; tx < G.tiles_x; ++tx)
+                for (uint32_t i : G.views)
+                    q[(uint32_t)G.key[1] * kQueues + band].push_back((i << 20) | (ty * G.tiles_x + tx));
+        }
     // one LDS layout per kernel: every wave's stack region is sized for the
     // deepest tree among the views that kernel drains
     if (!b->tuning.class_kernels) {
@@ -407,6 +432,7 @@ int xd_batch_set_tuning(xd_batch* b, const xd_batch_tuning* t) {
     if (t->waves_per_simd != 0 && t->waves_per_simd != 8)
         return fail(XD_ERR_INVALID_ARGUMENT, "waves_per_simd must be 0 (compiler's choice) or 8");
     if (t->class_kernels > 1) return fail(XD_ERR_INVALID_ARGUMENT, "class_kernels must be 0 or 1");
+    if (t->interleave > 1) return fail(XD_ERR_INVALID_ARGUMENT, "interleave must be 0 or 1");
     b->tuning = *t;
     if (b->views.empty()) return XD_OK;
     hipError_t e = hipSetDevice(b->ctx->device);
@@ -483,6 +509,24 @@ int xd_batch_autotune(xd_batch* b, uint32_t trials, xd_batch_tuning* chosen) {
         }
     }
     b->tuning.tiles_per_grab = best.tiles_per_grab;
+    // tile order: view by view, or interleaved across views of one group
+    best_ms = 1e30;
+    for (uint32_t k : {0u, 1u}) {
+        b->tuning.interleave = k;
+        int rc = prepare(b);
+        if (rc) return rc;
+        double m = 1e30;
+        for (uint32_t r = 0; r < trials; ++r) {
+            double ms = 0;
+            if ((rc = run(b, &ms))) return rc;
+            m = std::min(m, ms);
+        }
+        if (m < best_ms) {
+            best_ms = m;
+            best.interleave = k;
+        }
+    }
+    b->tuning.interleave = best.interleave;
     // one kernel for every traversal, or one per traversal
     best_ms = 1e30;
     for (uint32_t k : {0u, 1u}) {
