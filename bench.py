@@ -361,8 +361,9 @@ def main():
         svols = {"ct512_grid": ("grid", grid, [512, 361, 512]), "ct512_svo": ("svo", (nodes, side), [side] * 3),
                  "shells128_grid": ("grid", shells, [128] * 3),
                  "shells128_svo": ("svo", (snodes, sst["side"]), [sst["side"]] * 3)}
-        # served steps are small: one kernel for every traversal per step
-        serve_tuning = dict(tuning, class_kernels=0)
+        # served steps are small and mixed: 8x8 tiles, one tile per grab, one
+        # kernel for every traversal per step
+        serve_tuning = {"tile_w": [8] * 5, "tiles_per_grab": 1, "waves_per_simd": 0, "class_kernels": 0}
         for k, r in enumerate(rates):
             res = served_leg(ctx, svols, r, args.serve_seconds, seed=1000 + k, tuning=serve_tuning)
             log(f"served @ {r:.0f} req/s: {res['mrays_s']} Mrays/s, p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms")

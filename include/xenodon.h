@@ -323,6 +323,12 @@ typedef struct {
  * malloc'd into *out_nodes and released with xd_free. */
 XD_API int xd_build_octree(const void* rgba8, uint64_t dim_x, uint64_t dim_y, uint64_t dim_z, int heuristic,
                            int chan_diff, double std_dev, int tree_type, uint32_t** out_nodes, xd_build_stats* stats);
+/* The same build on the GPU from a grid volume already in HBM, for the
+ * channel-difference heuristic and sparse or rope trees (DAGs and the std-dev
+ * heuristic: xd_build_octree).  The node buffer and stats are identical to
+ * xd_build_octree's; the buffer is malloc'd into *out_nodes (xd_free). */
+XD_API int xd_build_octree_device(xd_context* ctx, const xd_volume* grid, int chan_diff, int tree_type,
+                                  uint32_t** out_nodes, xd_build_stats* stats);
 XD_API void xd_free(void* p);
 
 #ifdef __cplusplus

@@ -137,6 +137,7 @@ def _declare(L):
         "xd_build_octree": ([vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
                              ctypes.c_double, ctypes.c_int, P(vp), P(BuildStats)], ctypes.c_int),
         "xd_free": ([vp], None),
+        "xd_build_octree_device": ([vp, vp, ctypes.c_int, ctypes.c_int, P(vp), P(BuildStats)], ctypes.c_int),
         "xd_batch_create": ([vp, P(vp)], ctypes.c_int),
         "xd_batch_set_views": ([vp, P(View), ctypes.c_uint32], ctypes.c_int),
         "xd_batch_set_tuning": ([vp, P(BatchTuning)], ctypes.c_int),
@@ -281,6 +282,15 @@ class Context:
         h = ctypes.c_void_p()
         check(lib().xd_server_create(self.h, ctypes.byref(c), ctypes.byref(h)))
         return self._adopt(Server(self, h))
+
+    def build_octree(self, grid_volume, chan_diff=0, tree=None):
+        """GPU octree build from a grid volume in HBM (sparse or rope trees,
+        channel-difference heuristic); same (nodes, stats) as build_octree."""
+        out = ctypes.c_void_p()
+        st = BuildStats()
+        check(lib().xd_build_octree_device(self.h, grid_volume.h, chan_diff, TREE_SPARSE if tree is None else tree,
+                                           ctypes.byref(out), ctypes.byref(st)))
+        return _take_nodes(out, st)
 
     def set_tile(self, tile_w):
         check(lib().xd_context_set_tile(self.h, tile_w))
@@ -522,6 +532,10 @@ def build_octree(grid, heuristic=HEURISTIC_CHAN_DIFF, chan_diff=0, std_dev=0.0, 
     st = BuildStats()
     check(lib().xd_build_octree(g.ctypes.data, g.shape[2], g.shape[1], g.shape[0], heuristic, chan_diff,
                                 float(std_dev), tree, ctypes.byref(out), ctypes.byref(st)))
+    return _take_nodes(out, st)
+
+
+def _take_nodes(out, st):
     n = st.node_count
     try:
         buf = (ctypes.c_uint32 * (n * 10)).from_address(out.value)

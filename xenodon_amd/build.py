@@ -20,7 +20,7 @@ LIB = os.path.join(PKG, "libxenodon_amd.so")
 BIN_DIR = os.path.join(PKG, "bin")
 ARCH = os.environ.get("XD_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["xd_kernels.hip", "xd_runtime.hip", "xd_batch.hip", "xd_serve.hip"]
+HIP_SOURCES = ["xd_kernels.hip", "xd_runtime.hip", "xd_batch.hip", "xd_serve.hip", "xd_build.hip"]
 HOST_SOURCES = ["xd_convert.cpp", "xd_layout.cpp"]
 
 # IEEE-strict float semantics (bit parity with the CPU restatement), and no
