@@ -195,6 +195,8 @@ def main():
     ap.add_argument("--per-traversal-groups", type=int, default=3,
                     help="launches timed per traversal for the per-traversal batched rates (0: skip)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default="", help="batch tuning fields as JSON, applied over the defaults "
+                                                 "(with --no-autotune)")
     ap.add_argument("--traversals", default=",".join(TRAV_ORDER))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8, help="camera frames in the CPU-baseline sample")
@@ -292,7 +294,19 @@ def main():
                 views.append((vol, abi.TRAVERSALS[name], cam, uni(md), frames_pool[j * len(travs) + i]))
         return views
 
-    batch.set_views(group_views(0))
+    # tune on frames spread over the whole path (orbit, zoom and inside)
+    tune_views = []
+    for j in range(B):
+        f, up, eye = frame_of((j * args.steps) // B)
+        cam = abi.make_camera(f, up, eye)
+        for i, name in enumerate(travs):
+            vol, md = vols[name]
+            tune_views.append((vol, abi.TRAVERSALS[name], cam, uni(md), frames_pool[j * len(travs) + i]))
+    batch.set_views(tune_views)
+    if args.tuning:
+        t = dict(batch.tuning(), **json.loads(args.tuning))
+        batch.set_tuning(t["tile_w"], t["tiles_per_grab"], t["waves_per_simd"], t["class_kernels"], t["interleave"],
+                         t["mix_classes"])
     tuning = batch.autotune(trials=2) if not args.no_autotune else batch.tuning()
     log(f"batched: tuning {tuning}")
     for w in range(max(1, args.warmup // B)):
@@ -321,7 +335,7 @@ def main():
         vol, md = vols[name]
         sub = ctx.batch()
         sub.set_tuning(tuning["tile_w"], tuning["tiles_per_grab"], tuning["waves_per_simd"], tuning["class_kernels"],
-                       tuning["interleave"])
+                       tuning["interleave"], tuning["mix_classes"])
         ms_sum = 0.0
         rays_sum = 0
         n_groups = min(len(groups), args.per_traversal_groups)

@@ -190,9 +190,12 @@ typedef struct {
  * choice, 8 = capped for 8 waves per SIMD), and whether one persistent kernel
  * drains every traversal's tiles (class_kernels = 0) or each traversal gets
  * its own kernel, with registers and LDS of that traversal only (1), and the
- * tile order: 0 = view by view, 1 = views of the same volume, traversal and
- * size interleaved tile by tile, so that the same screen tile of, e.g.,
- * consecutive camera frames is traced at the same time through the same L2.
+ * tile order: 0 = view by view, rows of tiles; 1 = views of the same volume,
+ * traversal and size interleaved tile by tile, so that the same screen tile
+ * of, e.g., consecutive camera frames is traced at the same time through the
+ * same L2; 2, 3, 4 = view by view in squares of 8x8, 4x4, 16x16 tiles, so the
+ * tiles in flight cover a compact region of the image (a narrow cone of
+ * rays); 5 = interleaved as 1, in squares of 8x8 tiles.
  * Results never depend on it. */
 typedef struct {
     uint32_t tile_w[5];
@@ -200,6 +203,8 @@ typedef struct {
     uint32_t waves_per_simd;
     uint32_t class_kernels;
     uint32_t interleave;
+    uint32_t mix_classes; /* all-traversal kernel: 1 = its waves start on different
+                             traversals, so each CU runs a mix of them at once */
 } xd_batch_tuning;
 
 typedef struct xd_batch xd_batch;

@@ -70,7 +70,7 @@ class View(ctypes.Structure):
 class BatchTuning(ctypes.Structure):
     _fields_ = [("tile_w", ctypes.c_uint32 * 5), ("tiles_per_grab", ctypes.c_uint32),
                 ("waves_per_simd", ctypes.c_uint32), ("class_kernels", ctypes.c_uint32),
-                ("interleave", ctypes.c_uint32)]
+                ("interleave", ctypes.c_uint32), ("mix_classes", ctypes.c_uint32)]
 
 
 class BuildStats(ctypes.Structure):
@@ -279,6 +279,7 @@ class Context:
             c.tuning.waves_per_simd = tuning.get("waves_per_simd", 0)
             c.tuning.class_kernels = tuning.get("class_kernels", 0)
             c.tuning.interleave = tuning.get("interleave", 0)
+            c.tuning.mix_classes = tuning.get("mix_classes", 0)
         h = ctypes.c_void_p()
         check(lib().xd_server_create(self.h, ctypes.byref(c), ctypes.byref(h)))
         return self._adopt(Server(self, h))
@@ -350,22 +351,23 @@ class Batch:
         t = BatchTuning()
         check(lib().xd_batch_get_tuning(self.h, ctypes.byref(t)))
         return {"tile_w": list(t.tile_w), "tiles_per_grab": t.tiles_per_grab, "waves_per_simd": t.waves_per_simd,
-                "class_kernels": t.class_kernels, "interleave": t.interleave}
+                "class_kernels": t.class_kernels, "interleave": t.interleave, "mix_classes": t.mix_classes}
 
-    def set_tuning(self, tile_w, tiles_per_grab=1, waves_per_simd=0, class_kernels=0, interleave=0):
+    def set_tuning(self, tile_w, tiles_per_grab=1, waves_per_simd=0, class_kernels=0, interleave=0, mix_classes=0):
         t = BatchTuning()
         t.tile_w[:] = list(tile_w)
         t.tiles_per_grab = tiles_per_grab
         t.waves_per_simd = waves_per_simd
         t.class_kernels = class_kernels
         t.interleave = interleave
+        t.mix_classes = mix_classes
         check(lib().xd_batch_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, trials=3):
         t = BatchTuning()
         check(lib().xd_batch_autotune(self.h, trials, ctypes.byref(t)))
         return {"tile_w": list(t.tile_w), "tiles_per_grab": t.tiles_per_grab, "waves_per_simd": t.waves_per_simd,
-                "class_kernels": t.class_kernels, "interleave": t.interleave}
+                "class_kernels": t.class_kernels, "interleave": t.interleave, "mix_classes": t.mix_classes}
 
     def render(self):
         """Returns (ms, rays)."""

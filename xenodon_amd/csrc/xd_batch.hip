@@ -81,15 +81,18 @@ __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, ui
 template <int TRAV>
 __device__ __forceinline__ void class_body(const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles,
                                            const uint32_t* __restrict__ qbeg, uint32_t* __restrict__ counters,
-                                           uint32_t per_grab, uint32_t* lds) {
+                                           uint32_t per_grab, uint32_t mix, uint32_t* lds) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t home = blockIdx.x % kQueues;
     constexpr uint32_t kRanges = TRAV == (int)kClasses ? kClasses * kQueues : kQueues;
+    // mixed: consecutive waves of a block start on different classes
+    const uint32_t first = mix ? (blockIdx.x / kQueues * 4u + (threadIdx.x >> 6)) % kClasses : 0u;
     for (uint32_t qi = 0; qi < kRanges; ++qi) {
         // this XCD's range first (of every class, for the all-class kernel),
         // then the next XCD's
-        const uint32_t qq = TRAV == (int)kClasses ? (qi % kClasses) * kQueues + (home + qi / kClasses) % kQueues
-                                                  : (home + qi) % kQueues;
+        const uint32_t qq = TRAV == (int)kClasses
+                                ? ((qi + first) % kClasses) * kQueues + (home + qi / kClasses) % kQueues
+                                : (home + qi) % kQueues;
         const uint32_t beg = qbeg[qq], end = qbeg[qq + 1];
         for (;;) {
             uint32_t t = 0;
@@ -113,22 +116,22 @@ __device__ __forceinline__ void class_body(const BatchView* __restrict__ views, 
 template <int TRAV>
 __global__ __launch_bounds__(256) void k_batch(const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles,
                                                const uint32_t* __restrict__ qbeg, uint32_t* __restrict__ counters,
-                                               uint32_t per_grab) {
+                                               uint32_t per_grab, uint32_t mix) {
     extern __shared__ uint32_t lds[];
-    class_body<TRAV>(views, tiles, qbeg, counters, per_grab, lds);
+    class_body<TRAV>(views, tiles, qbeg, counters, per_grab, mix, lds);
 }
 
 // ... or capped for 8 waves per SIMD (registers beyond the cap spill).
 template <int TRAV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_w8(
     const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ qbeg,
-    uint32_t* __restrict__ counters, uint32_t per_grab) {
+    uint32_t* __restrict__ counters, uint32_t per_grab, uint32_t mix) {
     extern __shared__ uint32_t lds[];
-    class_body<TRAV>(views, tiles, qbeg, counters, per_grab, lds);
+    class_body<TRAV>(views, tiles, qbeg, counters, per_grab, mix, lds);
 }
 
 namespace {
-using BatchKernel = void (*)(const BatchView*, const uint32_t*, const uint32_t*, uint32_t*, uint32_t);
+using BatchKernel = void (*)(const BatchView*, const uint32_t*, const uint32_t*, uint32_t*, uint32_t, uint32_t);
 const BatchKernel kKernels[2][kClasses + 1] = {
     {k_batch<0>, k_batch<1>, k_batch<2>, k_batch<3>, k_batch<4>, k_batch<kClasses>},
     {k_batch_w8<0>, k_batch_w8<1>, k_batch_w8<2>, k_batch_w8<3>, k_batch_w8<4>, k_batch_w8<kClasses>}};
@@ -173,14 +176,14 @@ int batch_grid(int device, int traversal, uint32_t slots, bool w8, uint32_t* gri
 }
 
 hipError_t launch_batch(const BatchView* views, const uint32_t* tiles, const uint32_t* qbeg, uint32_t* counters,
-                        uint32_t grab, const BatchClasses& cls, bool w8, hipStream_t s) {
+                        uint32_t grab, const BatchClasses& cls, bool w8, bool mix, hipStream_t s) {
     grab = std::max<uint32_t>(1, std::min<uint32_t>(grab, 8));
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counters);
     for (int c = 0; c <= (int)kClasses; ++c) {
         if (!cls.grid[c]) continue;  // no tiles for this kernel
         const uint32_t base = c < (int)kClasses ? c * kQueues : 0;
         hipLaunchKernelGGL(kKernels[w8][c], dim3(cls.grid[c]), dim3(256), batch_lds_bytes(cls.slots[c]), s, views,
-                           tiles, qbeg + base, counters + base, grab);
+                           tiles, qbeg + base, counters + base, grab, mix ? 1u : 0u);
     }
     return hipGetLastError();
 }
@@ -204,7 +207,7 @@ struct xd_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     BatchClasses graph_cls{};  // per-class grids and LDS stack strides the graph was captured with
-    uint32_t graph_grab = 0, graph_w8 = 0;
+    uint32_t graph_grab = 0, graph_w8 = 0, graph_mix = 0;
 };
 
 namespace {
@@ -267,7 +270,8 @@ int upload_tables(xd_batch* b, BatchClasses* need) {
         const uint32_t c = (uint32_t)v.traversal;
         cls.slots[c] = std::max(cls.slots[c], batch_view_slots(B));
         cls.grid[c] = 1;
-        if (b->tuning.interleave) {
+        const uint32_t order = b->tuning.interleave;
+        if (order == 1 || order == 5) {
             // views of one volume, traversal, size and tile shape form a group
             // whose tiles are dealt together below
             const uint64_t key[6] = {(uint64_t)(uintptr_t)v.volume, (uint64_t)c, W, H, B.tw_shift, 0};
@@ -276,21 +280,31 @@ int upload_tables(xd_batch* b, BatchClasses* need) {
             if (g == groups.size()) groups.push_back({{key[0], key[1], key[2], key[3], key[4], key[5]}, {}, B.tiles_x, tiles_y});
             groups[g].views.push_back(i);
         } else {
-            for (uint32_t ty = 0; ty < tiles_y; ++ty) {
-                const uint32_t band = (uint32_t)((uint64_t)ty * kQueues / tiles_y);
-                for (uint32_t tx = 0; tx < B.tiles_x; ++tx)
-                    q[c * kQueues + band].push_back((i << 20) | (ty * B.tiles_x + tx));
+            // squares of sq x sq tiles (sq = 1: plain rows), row-major within
+            // each band's squares
+            const uint32_t sq = order == 2 ? 8u : order == 3 ? 4u : order == 4 ? 16u : 1u;
+            for (uint32_t ty0 = 0; ty0 < tiles_y; ty0 += sq) {
+                const uint32_t band = (uint32_t)((uint64_t)ty0 * kQueues / tiles_y);
+                const uint32_t ty1 = std::min(tiles_y, ty0 + sq);
+                for (uint32_t tx0 = 0; tx0 < B.tiles_x; tx0 += sq)
+                    for (uint32_t ty = ty0; ty < ty1; ++ty)
+                        for (uint32_t tx = tx0; tx < std::min(B.tiles_x, tx0 + sq); ++tx)
+                            q[c * kQueues + band].push_back((i << 20) | (ty * B.tiles_x + tx));
             }
         }
         b->total_rays += (uint64_t)W * H;
     }
     // interleaved: the same tile of every view of a group, one after another
+    // (order 5: in squares of 8 x 8 tiles)
+    const uint32_t gsq = b->tuning.interleave == 5 ? 8u : 1u;
     for (const auto& G : groups)
-        for (uint32_t ty = 0; ty < G.tiles_y; ++ty) {
-            const uint32_t band = (uint32_t)((uint64_t)ty * kQueues / G.tiles_y);
-            for (uint32_t tx = 0; tx < G.tiles_x; ++tx)
-                for (uint32_t i : G.views)
-                    q[(uint32_t)G.key[1] * kQueues + band].push_back((i << 20) | (ty * G.tiles_x + tx));
+        for (uint32_t ty0 = 0; ty0 < G.tiles_y; ty0 += gsq) {
+            const uint32_t band = (uint32_t)((uint64_t)ty0 * kQueues / G.tiles_y);
+            for (uint32_t tx0 = 0; tx0 < G.tiles_x; tx0 += gsq)
+                for (uint32_t ty = ty0; ty < std::min(G.tiles_y, ty0 + gsq); ++ty)
+                    for (uint32_t tx = tx0; tx < std::min(G.tiles_x, tx0 + gsq); ++tx)
+                        for (uint32_t i : G.views)
+                            q[(uint32_t)G.key[1] * kQueues + band].push_back((i << 20) | (ty * G.tiles_x + tx));
         }
     // one LDS layout per kernel: every wave's stack region is sized for the
     // deepest tree among the views that kernel drains
@@ -341,7 +355,8 @@ int capture(xd_batch* b, const BatchClasses& need) {
     const uint32_t grab = std::max<uint32_t>(1, std::min<uint32_t>(b->tuning.tiles_per_grab, 8));
     if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return fail_hip(e, "hipStreamBeginCapture");
-    const hipError_t le = launch_batch(b->d_views, b->d_tiles, b->d_qbeg, b->d_counters, grab, cls, w8, s);
+    const hipError_t le =
+        launch_batch(b->d_views, b->d_tiles, b->d_qbeg, b->d_counters, grab, cls, w8, b->tuning.mix_classes != 0, s);
     if ((e = hipStreamEndCapture(s, &b->graph)) != hipSuccess) return fail_hip(e, "hipStreamEndCapture");
     if (le != hipSuccess) return fail_hip(le, "batch launch capture");
     if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
@@ -349,6 +364,7 @@ int capture(xd_batch* b, const BatchClasses& need) {
     b->graph_cls = cls;
     b->graph_grab = grab;
     b->graph_w8 = w8;
+    b->graph_mix = b->tuning.mix_classes;
     return XD_OK;
 }
 
@@ -357,7 +373,8 @@ int prepare(xd_batch* b) {
     int rc = upload_tables(b, &need);
     if (rc) return rc;
     const uint32_t grab = std::max<uint32_t>(1, std::min<uint32_t>(b->tuning.tiles_per_grab, 8));
-    bool stale = !b->exec || grab != b->graph_grab || b->graph_w8 != (uint32_t)(b->tuning.waves_per_simd == 8);
+    bool stale = !b->exec || grab != b->graph_grab || b->graph_w8 != (uint32_t)(b->tuning.waves_per_simd == 8) ||
+                 b->graph_mix != b->tuning.mix_classes;
     for (int c = 0; c <= (int)kClasses; ++c)
         stale |= (need.grid[c] && !b->graph_cls.grid[c]) || need.slots[c] > b->graph_cls.slots[c];
     return stale ? capture(b, need) : XD_OK;
@@ -432,7 +449,8 @@ int xd_batch_set_tuning(xd_batch* b, const xd_batch_tuning* t) {
     if (t->waves_per_simd != 0 && t->waves_per_simd != 8)
         return fail(XD_ERR_INVALID_ARGUMENT, "waves_per_simd must be 0 (compiler's choice) or 8");
     if (t->class_kernels > 1) return fail(XD_ERR_INVALID_ARGUMENT, "class_kernels must be 0 or 1");
-    if (t->interleave > 1) return fail(XD_ERR_INVALID_ARGUMENT, "interleave must be 0 or 1");
+    if (t->interleave > 5) return fail(XD_ERR_INVALID_ARGUMENT, "interleave (tile order) must be 0..5");
+    if (t->mix_classes > 1) return fail(XD_ERR_INVALID_ARGUMENT, "mix_classes must be 0 or 1");
     b->tuning = *t;
     if (b->views.empty()) return XD_OK;
     hipError_t e = hipSetDevice(b->ctx->device);
@@ -509,9 +527,9 @@ int xd_batch_autotune(xd_batch* b, uint32_t trials, xd_batch_tuning* chosen) {
         }
     }
     b->tuning.tiles_per_grab = best.tiles_per_grab;
-    // tile order: view by view, or interleaved across views of one group
+    // tile order: rows, interleaved across views of one group, or squares
     best_ms = 1e30;
-    for (uint32_t k : {0u, 1u}) {
+    for (uint32_t k : {0u, 1u, 2u, 3u, 4u, 5u}) {
         b->tuning.interleave = k;
         int rc = prepare(b);
         if (rc) return rc;
@@ -527,6 +545,24 @@ int xd_batch_autotune(xd_batch* b, uint32_t trials, xd_batch_tuning* chosen) {
         }
     }
     b->tuning.interleave = best.interleave;
+    // all-traversal kernel: waves on one class at a time, or a mix
+    best_ms = 1e30;
+    for (uint32_t k : {0u, 1u}) {
+        b->tuning.mix_classes = k;
+        int rc = prepare(b);
+        if (rc) return rc;
+        double m = 1e30;
+        for (uint32_t r = 0; r < trials; ++r) {
+            double ms = 0;
+            if ((rc = run(b, &ms))) return rc;
+            m = std::min(m, ms);
+        }
+        if (m < best_ms) {
+            best_ms = m;
+            best.mix_classes = k;
+        }
+    }
+    b->tuning.mix_classes = best.mix_classes;
     // one kernel for every traversal, or one per traversal
     best_ms = 1e30;
     for (uint32_t k : {0u, 1u}) {

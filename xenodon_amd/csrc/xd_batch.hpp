@@ -54,7 +54,8 @@ struct BatchClasses {
 // Enqueues the counter reset and the present kernels of cls on s.  qbeg holds
 // kClasses * kQueues + 1 range starts (class c's ranges are entries
 // c*kQueues .. c*kQueues+kQueues); counters holds kClasses * kQueues.
+// mix: the all-class kernel's waves start on different traversal classes.
 hipError_t launch_batch(const BatchView* views, const uint32_t* tiles, const uint32_t* qbeg, uint32_t* counters,
-                        uint32_t grab, const BatchClasses& cls, bool w8, hipStream_t s);
+                        uint32_t grab, const BatchClasses& cls, bool w8, bool mix, hipStream_t s);
 
 }  // namespace xd

@@ -154,6 +154,7 @@ struct xd_server {
     uint32_t grab = 1;
     bool w8 = false;
     bool class_kernels = false;
+    bool mix = false;
 
     // state
     bool stop = false, reader_stop = false;
@@ -292,7 +293,7 @@ bool launch_step(xd_server* s, Step& S, std::string* err) {
                           reinterpret_cast<const uint32_t*>(S.dev + s->tiles_off),
                           reinterpret_cast<const uint32_t*>(S.dev + s->qbeg_off),
                           reinterpret_cast<uint32_t*>(S.dev + s->counters_off), s->grab, cls, s->w8,
-                          S.stream)) != hipSuccess ||
+                          s->mix, S.stream)) != hipSuccess ||
         (e = hipEventRecord(S.ev1, S.stream)) != hipSuccess) {
         *err = std::string("served step launch: ") + hipGetErrorString(e);
         return false;
@@ -496,6 +497,7 @@ int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** o
     s->grab = c.tuning.tiles_per_grab;
     s->w8 = c.tuning.waves_per_simd == 8;
     s->class_kernels = c.tuning.class_kernels != 0;
+    s->mix = c.tuning.mix_classes != 0;
     s->ring = c.launches_in_flight;
     // table layout of one step: views | qbeg | counters | tiles
     s->views_off = 0;
