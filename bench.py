@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run a device copy of this size (HBM counter calibration)")
     ap.add_argument("--tile", type=int, default=0, help="per-wave tile width override (8/16/32)")
+    ap.add_argument("--xcd-chunks", type=int, default=-1,
+                    help="plain launches: image block runs per XCD (0: hardware order; default: the library's)")
     ap.add_argument("--serve-rates", default="200,1000,2000", help="Poisson request rates (req/s); empty: skip")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-saturated", type=int, default=400,
@@ -229,6 +231,8 @@ def main():
     ctx = abi.Context(0)
     if args.tile:
         ctx.set_tile(args.tile)
+    if args.xcd_chunks >= 0:
+        ctx.set_option("xcd_chunks", args.xcd_chunks)
     gvol = ctx.grid_volume(grid)
     svol = ctx.svo_volume(nodes, side)
     frame = ctx.frame(WIDTH, HEIGHT)

@@ -119,7 +119,12 @@ XD_API int xd_context_set_tile(xd_context* ctx, uint32_t tile_w);
  *   "octree_layout"  0 (default) derive the tagged device layout for trees of
  *                    at most 2^30 nodes; 1 keep the .svo node words as they
  *                    are (the layout used above 2^30 nodes).  Applies to
- *                    volumes created afterwards; results are identical. */
+ *                    volumes created afterwards; results are identical.
+ *   "xcd_chunks"     plain launches (xd_render): 0 (default) leaves the
+ *                    image's pixel blocks in dispatch order, so neighbouring
+ *                    blocks run on different XCDs; N in [1, 64] cuts the
+ *                    blocks into 8N runs dealt to the 8 XCDs in turn (N = 1:
+ *                    one contiguous band per XCD).  Results are identical. */
 XD_API int xd_context_set_option(xd_context* ctx, const char* name, int64_t value);
 
 /* ---- volumes (replaces Dda/SvoRaytraceResources, src/render/) ------------ */

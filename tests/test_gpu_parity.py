@@ -140,6 +140,29 @@ def test_output_regions_stitch_to_display(ctx, c1_scene, t):
 
 
 @pytest.mark.parametrize("t", TRAVERSALS)
+def test_block_order_and_tile_shape_keep_frames_bit_exact(ctx, c1_scene, t):
+    """Plain launches in every XCD block order (dispatch order, one band per
+    XCD, interleaved runs) and every wave tile shape give the oracle's frame
+    (200x120 leaves partial blocks and runs at the image's end)."""
+    f, up, eye = c1_orbit()[6]
+    ov, gv, md = c1_scene.pick(t)
+    ref = xo.render(t, ov, xo.make_camera(f, up, eye), xo.make_uniforms(200, 120, md, emission=C1_EMISSION))["rgba"]
+    try:
+        for chunks in (0, 1, 3, 64):
+            for tile in (8, 16, 32):
+                ctx.set_option("xcd_chunks", chunks)
+                ctx.set_tile(tile)
+                img, _ = ctx.render_image(gv, t, abi.make_camera(f, up, eye),
+                                          abi.make_uniforms(200, 120, md, emission=C1_EMISSION))
+                _assert_parity(img, ref, f"order{chunks}/tile{tile}/{abi.TRAVERSAL_NAMES[t]}")
+    finally:
+        ctx.set_option("xcd_chunks", 0)
+        ctx.set_tile(8)
+    with pytest.raises(abi.XdError):
+        ctx.set_option("xcd_chunks", 65)
+
+
+@pytest.mark.parametrize("t", TRAVERSALS)
 def test_word_counts_match_oracle(ctx, small_scenes, c1_scene, t):
     """The instrumented kernel counts exactly the 32-bit volume words the
     reference algorithm reads (the roofline's algorithmic bytes)."""

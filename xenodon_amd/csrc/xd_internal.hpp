@@ -57,6 +57,7 @@ struct RenderParams {
     uint32_t node_count;
     uint32_t top_nodes;  // child words of nodes [0, top_nodes) are staged in LDS
     uint32_t depth_exact;  // every internal node's depth field equals its depth in the tree
+    uint32_t xcd_chunks;  // plain launches: block runs per XCD (0: hardware order)
     unsigned long long* words;  // word counter (instrumented variant only)
 };
 

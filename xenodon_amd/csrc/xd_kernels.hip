@@ -25,7 +25,7 @@ __global__ __launch_bounds__(256) void k_dda(RenderParams p) {
         __syncthreads();
     }
     uint32_t ix, iy;
-    block_pixel<TW, 4>(p.view.out_ext[0], p.view.out_ext[1], ix, iy);
+    block_pixel<TW, 4>(p.view.out_ext[0], p.view.out_ext[1], p.xcd_chunks, ix, iy);
     uint64_t words = 0;
     if (ix < p.view.out_ext[0] && iy < p.view.out_ext[1]) {
         f3 rd = ray_dir(p.cam, p.view, ix, iy);
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void k_svo(RenderParams p) {
         __syncthreads();
     }
     uint32_t ix, iy;
-    block_pixel<TW, 4>(p.view.out_ext[0], p.view.out_ext[1], ix, iy);
+    block_pixel<TW, 4>(p.view.out_ext[0], p.view.out_ext[1], p.xcd_chunks, ix, iy);
     uint64_t words = 0;
     if (ix < p.view.out_ext[0] && iy < p.view.out_ext[1]) {
         f3 rd = ray_dir(p.cam, p.view, ix, iy);

@@ -227,6 +227,11 @@ int xd_context_set_option(xd_context* ctx, const char* name, int64_t value) {
         return XD_OK;
     }
     if (strcmp(name, "tile_w") == 0) return xd_context_set_tile(ctx, (uint32_t)value);
+    if (strcmp(name, "xcd_chunks") == 0) {
+        if (value < 0 || value > 64) return fail(XD_ERR_INVALID_ARGUMENT, "xcd_chunks must be in [0, 64]");
+        ctx->xcd_chunks = (uint32_t)value;
+        return XD_OK;
+    }
     return fail(XD_ERR_INVALID_ARGUMENT, "unknown option '%s'", name);
 }
 
@@ -527,6 +532,7 @@ int xd_render(xd_context* ctx, const xd_volume* vol, int traversal, const xd_cam
     fill_params(p, vol, traversal, cam, u);
     p.out = frame->d_pixels;
     p.out_pitch = frame->w;
+    p.xcd_chunks = ctx->xcd_chunks;
     if ((e = hipEventRecord(ctx->ev0, ctx->stream)) != hipSuccess) return fail_hip(e, "hipEventRecord");
     if ((e = launch_traversal(traversal, ctx->tile_w, false, p, ctx->stream)) != hipSuccess)
         return fail_hip(e, "traversal launch");

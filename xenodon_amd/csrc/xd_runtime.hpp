@@ -13,6 +13,7 @@ struct xd_context {
     int device = 0;
     int tile_w = 8;  // per-wavefront tile width: 8 (8x8), 16 (16x4), 32 (32x2)
     int raw_octrees = 0;  // keep .svo words instead of the tagged layout
+    uint32_t xcd_chunks = 0;  // plain launches: block runs per XCD (see block_pixel)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long* d_words = nullptr;

@@ -358,6 +358,8 @@ __device__ __forceinline__ f3 fast_esvo(const FastSvo& s, uint64_t cap, uint32_t
     uint32_t idx = r.idx, parent = 0, scale = 22;
     float scale_exp2 = 0.5f;
     f3 total = splat(0.0f);
+    uint32_t pend_col = 0;  // the last leaf's term, added at the next leaf so that
+    float pend_w = 0.0f;    // its colour load overlaps the walk in between
     uint64_t it = 0;
     while (scale < 23u) {
         if (++it > cap) break;
@@ -369,7 +371,10 @@ __device__ __forceinline__ f3 fast_esvo(const FastSvo& s, uint64_t cap, uint32_t
                 const uint32_t wd = cw_at(s, parent, idx ^ r.octant_mask);
                 const uint32_t child = wd & kIndexMask;
                 if (wd & kTagLeaf) {
-                    total = add(total, muls(unorm_rgb(colour_if(s, child, wd)), tv_max - t_min));
+                    const uint32_t col = colour_if(s, child, wd);
+                    total = add(total, muls(unorm_rgb(pend_col), pend_w));  // +0 the first time
+                    pend_col = col;
+                    pend_w = tv_max - t_min;
                 } else {
                     if (tc_max < h) {
                         uint32_t slot = 22u - scale;
@@ -404,7 +409,7 @@ __device__ __forceinline__ f3 fast_esvo(const FastSvo& s, uint64_t cap, uint32_t
             h = 0.0f;
         }
     }
-    return total;
+    return add(total, muls(unorm_rgb(pend_col), pend_w));
 }
 
 }  // namespace xd

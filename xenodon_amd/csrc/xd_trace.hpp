@@ -8,17 +8,27 @@
 namespace xd {
 
 // Pixel of this lane.  WPB = 4: a block is a 2x2 arrangement of wavefront
-// tiles; WPB = 1: a block is one tile.  Blocks are renumbered so that each of
-// the 8 XCDs (which receive blocks round-robin) renders one contiguous band.
+// tiles; WPB = 1: a block is one tile.  The 8 XCDs receive blocks
+// round-robin (block b runs on XCD b % 8).  chunks = 0 keeps that order, so
+// neighbouring blocks run on different XCDs; chunks = N >= 1 cuts the image's
+// blocks (row-major) into 8N runs and gives XCD x runs x, x + 8, ..., so each
+// XCD's L2 holds the volume behind a few bands while every XCD still gets a
+// share of each part of the image (N = 1: one contiguous band per XCD).
 template <int TW, int WPB>
-__device__ __forceinline__ void block_pixel(uint32_t W, uint32_t H, uint32_t& ix, uint32_t& iy) {
+__device__ __forceinline__ void block_pixel(uint32_t W, uint32_t H, uint32_t chunks, uint32_t& ix, uint32_t& iy) {
     constexpr uint32_t TH = 64 / TW;
     constexpr uint32_t BW = (WPB == 4 ? 2 : 1) * TW, BH = (WPB == 4 ? 2 : 1) * TH;
     const uint32_t nbx = (W + BW - 1) / BW;
     const uint32_t nblocks = nbx * ((H + BH - 1) / BH);
     const uint32_t b = blockIdx.x;
-    const uint32_t full = nblocks & ~7u;
-    const uint32_t lb = b < full ? (b & 7u) * (full >> 3) + (b >> 3) : b;
+    uint32_t lb = b;
+    if (chunks != 0) {
+        const uint32_t run = nblocks / (8u * chunks);  // blocks per run
+        if (run != 0 && b < run * 8u * chunks) {
+            const uint32_t j = b >> 3;  // j-th block this XCD receives
+            lb = ((j / run) * 8u + (b & 7u)) * run + j % run;
+        }
+    }
     const uint32_t bx = lb % nbx, by = lb / nbx;
     const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     ix = bx * BW + (WPB == 4 ? (w & 1u) * TW : 0u) + (lane % TW);
