@@ -12,10 +12,16 @@ octree).  Inputs are HBM-resident before timing.
   batched  `batch_frames` steps (x 5 traversals) per persistent-kernel
            hipGraph launch.  `value` = total rays / wall time of the timed
            batched steps.
+  mixed    64 views (dda / svo-df / esvo, sizes log-uniform in 256..1920 x
+           256..1080, orbit cameras) over the resident 128^3, 512x361x512
+           and 1024^3 volumes in one batched launch, against the plain form
+           on the same views.
   served   Poisson request arrivals (seeded cameras, log-uniform sizes from
            128x128 to 1920x1080, traversal uniform over all five) against
            the resident 128^3 and 512x361x512 volumes; p50/p99 latency from
-           arrival to the frame in host memory, and aggregate Mrays/s.
+           arrival to the frame in host memory, and aggregate Mrays/s; for the
+           requests released at once, also the plain form on the same
+           requests.
 
 The roofline object covers the dominant kernel: algorithmic bytes (4 B per
 volume word the reference algorithm reads, counted by the instrumented kernel
@@ -35,7 +41,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from xenodon_amd import abi, synth  # noqa: E402
+from xenodon_amd import abi, synth, workloads  # noqa: E402
 
 CAMERA_FILE = os.path.join(ROOT, "tests", "golden", "camera.txt")
 WIDTH, HEIGHT = 1920, 1080
@@ -101,40 +107,82 @@ def cpu_baseline(grid, nodes, side, cams, frames, threads, budget_s=30.0):
     }
 
 
-def look_at(eye, target, world_up=(0.0, 1.0, 0.0)):
-    eye, target = np.asarray(eye, np.float64), np.asarray(target, np.float64)
-    f = target - eye
-    f /= np.linalg.norm(f)
-    up = np.asarray(world_up, np.float64)
-    up = up - f * np.dot(up, f)
-    up /= np.linalg.norm(up)
-    return f, up, eye
+def plain_same_requests(ctx, volumes, reqs):
+    """The plain form on a served request mix: one launch per request, one
+    after another; Mrays/s over the summed launch times (the plain form's
+    definition) and over the wall time of the loop."""
+    vols = {}
+    for name, (kind, d, md) in volumes.items():
+        vols[name] = (ctx.grid_volume(d) if kind == "grid" else ctx.svo_volume(*d)), md
+    frames = {}
+    ms_sum, rays = 0.0, 0
+    t0 = time.perf_counter()
+    for name, t, cam, W, H in reqs:
+        vol, md = vols[name]
+        f = frames.get((W, H))
+        if f is None:
+            f = frames[(W, H)] = ctx.frame(W, H)
+        ms_sum += ctx.render(vol, t, abi.make_camera(*cam), abi.make_uniforms(W, H, md, emission=EMISSION), f)
+        rays += W * H
+    wall = time.perf_counter() - t0
+    for f in frames.values():
+        f.free()
+    for vol, _ in vols.values():
+        vol.free()
+    return {"requests": len(reqs), "mrays_s": round(rays / ms_sum / 1e3, 1),
+            "mrays_s_wall": round(rays / wall / 1e6, 1)}
 
 
-def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0):
+def mixed_views_leg(ctx, scenes, tuning, reps):
+    """The many-views batch: 64 views over the resident volumes (traversals
+    dda / svo-df / esvo, W x H log-uniform in 256..1920 x 256..1080, orbit
+    cameras at distance 0.5..3), one batched launch per repetition, against
+    the plain form on the same views (one launch per view).  Both rates are
+    over device time.  scenes: name -> (grid volume, grid model_dim, svo
+    volume, svo model_dim, emission)."""
+    spec = workloads.mixed_views(sorted(scenes))
+    views = []
+    for name, t, cam, W, H in spec:
+        gv, gmd, sv, smd, em = scenes[name]
+        vol, md = (gv, gmd) if t == abi.DDA else (sv, smd)
+        views.append((vol, t, abi.make_camera(*cam), abi.make_uniforms(W, H, md, emission=em), ctx.frame(W, H)))
+    rays = sum(W * H for *_, W, H in spec)
+    for v in views:  # warm
+        ctx.render(*v)
+    plain_ms = 0.0
+    for _ in range(reps):
+        for v in views:
+            plain_ms += ctx.render(*v)
+    b = ctx.batch()
+    b.set_tuning(tuning["tile_w"], tuning["tiles_per_grab"], tuning["waves_per_simd"], tuning["class_kernels"],
+                 tuning["interleave"], tuning["mix_classes"])
+    b.set_views(views)
+    b.render()  # warm
+    batch_ms = [b.render()[0] for _ in range(reps)]
+    b.free()
+    for v in views:
+        v[4].free()
+    per_trav = {abi.TRAVERSAL_NAMES[t]: sum(1 for _, tt, *_ in spec if tt == t) for t in (abi.DDA, abi.SVO_DF, abi.ESVO)}
+    batched = rays * reps / (sum(batch_ms) * 1e3)
+    plain = rays * reps / (plain_ms * 1e3)
+    return {"views": len(spec), "volumes": sorted(scenes), "views_per_traversal": per_trav, "rays_per_launch": rays,
+            "batched_mrays_s": round(batched, 1), "plain_mrays_s": round(plain, 1),
+            "batched_vs_plain": round(batched / plain, 3), "batched_launch_ms": round(float(np.mean(batch_ms)), 3)}
+
+
+def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plain=False):
     """Poisson arrivals at `rate` req/s for `duration_s` (or n_saturated
     requests all released at once).  volumes: name -> (kind, data, model_dim).
-    Returns latency percentiles (ms) and aggregate Mrays/s."""
+    Returns latency percentiles (ms) and aggregate Mrays/s; with plain, also
+    the plain form's rate on the same requests."""
     srv = ctx.server(tuning=tuning)
     ids = {}
     for name, (kind, d, md) in volumes.items():
         ids[name] = (srv.load_grid(d) if kind == "grid" else srv.load_svo(*d)), kind
-    grids = [n for n, (_, k) in ids.items() if k == "grid"]
-    svos = [n for n, (_, k) in ids.items() if k == "svo"]
-    rng = np.random.default_rng(seed)
     n = n_saturated if n_saturated else int(rate * duration_s)
+    named, rng = workloads.request_mix({k: v[1] for k, v in ids.items()}, n, seed)
     offsets = np.zeros(n) if n_saturated else np.cumsum(rng.exponential(1000.0 / rate, n))
-    reqs = []
-    for i in range(n):
-        t = int(rng.integers(5))
-        name = (grids if t == abi.DDA else svos)[int(rng.integers(len(grids if t == abi.DDA else svos)))]
-        W = int(round(np.exp(rng.uniform(np.log(128), np.log(1920)))))
-        H = int(round(np.exp(rng.uniform(np.log(128), np.log(1080)))))
-        ang = rng.uniform(0, 2 * np.pi)
-        elev = rng.uniform(-0.6, 0.6)
-        d = rng.uniform(0.5, 3.0)
-        eye = 0.5 + d * np.array([np.cos(ang) * np.cos(elev), np.sin(elev), np.sin(ang) * np.cos(elev)])
-        reqs.append((ids[name][0], t, abi.make_camera(*look_at(eye, (0.5, 0.5, 0.5))), W, H))
+    reqs = [(ids[name][0], t, abi.make_camera(*cam), W, H) for name, t, cam, W, H in named]
     t_base = srv.now_ms() + 20.0
     lead_ms = 10.0
 
@@ -161,14 +209,18 @@ def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0):
     dev = np.array([c["device_done_ms"] - c["arrival_ms"] for c in done])
     wall = max(c["done_ms"] for c in done) - min(c["arrival_ms"] for c in done)
     rays = sum(c["rays"] for c in done)
-    return {"rate_req_s": rate if not n_saturated else None, "requests": n, "duration_s": round(wall / 1e3, 3),
-            "mrays_s": round(rays / wall / 1e3, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
-            "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "p50_device_ms": round(float(np.percentile(dev, 50)), 3),
-            "p99_device_ms": round(float(np.percentile(dev, 99)), 3),
-            "mean_rays_per_request": round(rays / n), "launches": st["launches"],
-            "max_requests_per_launch": st["max_requests_per_launch"],
-            "device_busy_frac": round(st["busy_ms"] / wall, 3)}
+    res = {"rate_req_s": rate if not n_saturated else None, "requests": n, "duration_s": round(wall / 1e3, 3),
+           "mrays_s": round(rays / wall / 1e3, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+           "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "p50_device_ms": round(float(np.percentile(dev, 50)), 3),
+           "p99_device_ms": round(float(np.percentile(dev, 99)), 3),
+           "mean_rays_per_request": round(rays / n), "launches": st["launches"],
+           "max_requests_per_launch": st["max_requests_per_launch"],
+           "device_busy_frac": round(st["busy_ms"] / wall, 3)}
+    if plain:
+        res["plain_same_requests"] = plain_same_requests(ctx, volumes, named)
+        res["served_vs_plain"] = round(res["mrays_s"] / res["plain_same_requests"]["mrays_s"], 3)
+    return res
 
 
 def copy_calibration(gib):
@@ -194,9 +246,12 @@ def main():
     ap.add_argument("--batch-frames", type=int, default=10, help="camera frames per batched launch")
     ap.add_argument("--per-traversal-groups", type=int, default=3,
                     help="launches timed per traversal for the per-traversal batched rates (0: skip)")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the batch on frames spread over the path (default: the library's launch shape, "
+                         "chosen by sweeps on this workload)")
+    ap.add_argument("--no-autotune", action="store_true", help="the default; kept for older command lines")
     ap.add_argument("--tuning", default="", help="batch tuning fields as JSON, applied over the defaults "
-                                                 "(with --no-autotune)")
+                                                 "(without --autotune)")
     ap.add_argument("--traversals", default=",".join(TRAV_ORDER))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8, help="camera frames in the CPU-baseline sample")
@@ -205,8 +260,13 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="per-wave tile width override (8/16/32)")
     ap.add_argument("--xcd-chunks", type=int, default=-1,
                     help="plain launches: image block runs per XCD (0: hardware order; default: the library's)")
-    ap.add_argument("--serve-rates", default="200,1000,2000", help="Poisson request rates (req/s); empty: skip")
+    ap.add_argument("--mixed-views-reps", type=int, default=3,
+                    help="timed launches of the 64-view mixed batch (0: skip)")
+    ap.add_argument("--mixed-views-large", type=int, default=1,
+                    help="include the 1024^3 log-normal volume in the mixed batch (0: two volumes only)")
+    ap.add_argument("--serve-rates", default="200,1000,2000,3000", help="Poisson request rates (req/s); empty: skip")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-tuning", default="", help="served-form tuning fields as JSON, over the defaults")
     ap.add_argument("--serve-saturated", type=int, default=400,
                     help="requests released at once for the served form's peak rate (0: skip)")
     args = ap.parse_args()
@@ -311,7 +371,7 @@ def main():
         t = dict(batch.tuning(), **json.loads(args.tuning))
         batch.set_tuning(t["tile_w"], t["tiles_per_grab"], t["waves_per_simd"], t["class_kernels"], t["interleave"],
                          t["mix_classes"])
-    tuning = batch.autotune(trials=2) if not args.no_autotune else batch.tuning()
+    tuning = batch.autotune(trials=2) if args.autotune else batch.tuning()
     log(f"batched: tuning {tuning}")
     for w in range(max(1, args.warmup // B)):
         batch.set_views(group_views(w * B))
@@ -370,25 +430,53 @@ def main():
                 "mean_launch_ms": round(mean_batch_ms, 4),
                 "plain_per_traversal_algorithmic_GBs": plain_roof}
 
+    shells = synth.shells_128()
+    snodes, sst = abi.build_octree(shells, tree=abi.TREE_ROPE)
+
+    # ---------------- many views of three volumes in one batched launch ------
+    mixed = None
+    if args.mixed_views_reps > 0:
+        sg, ss = ctx.grid_volume(shells), ctx.svo_volume(snodes, sst["side"])
+        scenes = {"shells128": (sg, [128] * 3, ss, [sst["side"]] * 3, 4.0),
+                  "ct512": (gvol, [512, 361, 512], svol, [side] * 3, EMISSION)}
+        extra = []
+        if args.mixed_views_large:
+            t0 = time.time()
+            big = synth.lognormal(side=1024)
+            bg = ctx.grid_volume(big)
+            bnodes, bst = ctx.build_octree(bg, tree=abi.TREE_ROPE)
+            bs = ctx.svo_volume(bnodes, bst["side"])
+            del big, bnodes
+            scenes["lognormal1024"] = (bg, [1024] * 3, bs, [bst["side"]] * 3, 1.0)
+            extra = [bg, bs]
+            log(f"1024^3 log-normal grid + rope SVO ({bs.info().device_bytes / 1e9:.1f} GB) resident in "
+                f"{time.time() - t0:.1f}s")
+        mixed = mixed_views_leg(ctx, scenes, tuning, args.mixed_views_reps)
+        log(f"mixed views: batched {mixed['batched_mrays_s']} Mrays/s, plain {mixed['plain_mrays_s']} Mrays/s")
+        for v in [sg, ss] + extra:
+            v.free()
+
     # ---------------- served form: Poisson arrivals, continuous batching -----
     served = []
     rates = [float(r) for r in args.serve_rates.split(",") if r]
     if rates or args.serve_saturated:
-        shells = synth.shells_128()
-        snodes, sst = abi.build_octree(shells, tree=abi.TREE_ROPE)
         svols = {"ct512_grid": ("grid", grid, [512, 361, 512]), "ct512_svo": ("svo", (nodes, side), [side] * 3),
                  "shells128_grid": ("grid", shells, [128] * 3),
                  "shells128_svo": ("svo", (snodes, sst["side"]), [sst["side"]] * 3)}
         # served steps are small and mixed: 8x8 tiles, one tile per grab, one
         # kernel for every traversal per step
         serve_tuning = {"tile_w": [8] * 5, "tiles_per_grab": 1, "waves_per_simd": 0, "class_kernels": 0}
+        if args.serve_tuning:
+            serve_tuning.update(json.loads(args.serve_tuning))
         for k, r in enumerate(rates):
             res = served_leg(ctx, svols, r, args.serve_seconds, seed=1000 + k, tuning=serve_tuning)
             log(f"served @ {r:.0f} req/s: {res['mrays_s']} Mrays/s, p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms")
             served.append(res)
         if args.serve_saturated:
-            res = served_leg(ctx, svols, 0, 0, seed=999, tuning=serve_tuning, n_saturated=args.serve_saturated)
-            log(f"served, {args.serve_saturated} requests at once: {res['mrays_s']} Mrays/s, p99 {res['p99_ms']} ms")
+            res = served_leg(ctx, svols, 0, 0, seed=999, tuning=serve_tuning, n_saturated=args.serve_saturated,
+                             plain=True)
+            log(f"served, {args.serve_saturated} requests at once: {res['mrays_s']} Mrays/s, p99 {res['p99_ms']} ms; "
+                f"plain on the same requests: {res['plain_same_requests']['mrays_s']} Mrays/s")
             served.append(res)
 
     cpu = None
@@ -418,6 +506,7 @@ def main():
                   "per_traversal_mrays": {n: round(v, 1) for n, v in per_trav.items()},
                   "per_traversal_mean_ms": {n: round(v, 4) for n, v in mean_ms.items()}},
         "batched_per_traversal_mrays": {n: round(v, 1) for n, v in per_trav_batched.items()},
+        "mixed_views": mixed,
         "served": served,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -201,7 +201,9 @@ typedef struct {
  * same L2; 2, 3, 4 = view by view in squares of 8x8, 4x4, 16x16 tiles, so the
  * tiles in flight cover a compact region of the image (a narrow cone of
  * rays); 5 = interleaved as 1, in squares of 8x8 tiles.
- * Results never depend on it. */
+ * Results never depend on it.  A new batch starts at 8x8 tiles for every
+ * traversal, one tile per grab, 8 waves per SIMD, one kernel for every
+ * traversal with mixed classes, squares of 8x8 tiles. */
 typedef struct {
     uint32_t tile_w[5];
     uint32_t tiles_per_grab;
@@ -221,8 +223,11 @@ XD_API int xd_batch_create(xd_context* ctx, xd_batch** out);
 XD_API int xd_batch_set_views(xd_batch* batch, const xd_view* views, uint32_t count);
 XD_API int xd_batch_set_tuning(xd_batch* batch, const xd_batch_tuning* tuning);
 XD_API int xd_batch_get_tuning(const xd_batch* batch, xd_batch_tuning* tuning);
-/* Times every tile shape per traversal and every grab size on the current
- * views (trials runs each, best kept) and keeps the fastest. */
+/* Coordinate search from the current tuning on the current views (best of
+ * `trials` launches per setting, 0 = 3): the tile shape of each traversal on
+ * that traversal's views alone, then class mixing, tile order, register
+ * budget, per-class kernels and grab size on the whole batch.  A field
+ * changes only for a setting at least 2 % faster than the one it has. */
 XD_API int xd_batch_autotune(xd_batch* batch, uint32_t trials, xd_batch_tuning* chosen);
 /* Replays the batch's graph and waits.  *ms is the HIP-event time of the
  * whole launch; *rays the number of pixels rendered. */

@@ -2,14 +2,16 @@
 on its rope SVO (one allocation above 4 GB, built on the GPU) at 3840x2160
 over frames of the reference camera path.  Frames are rendered whole on the
 GPU and compared bit for bit with the oracle on bands of rows (the oracle
-renders the same output regions of the same display)."""
+renders the same output regions of the same display).  The many-views batch
+(64 views over the 128^3, 512x361x512 and 1024^3 volumes in one launch) is
+checked here too, since it needs the 1024^3 volumes resident."""
 import os
 
 import numpy as np
 import pytest
 
 from oracle import xo
-from xenodon_amd import abi, synth
+from xenodon_amd import abi, synth, workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -81,3 +83,55 @@ def test_c3_raw_octree_words_with_64bit_offsets(c3):
             a, _ = raw_ctx.render_image(rv, t, abi.make_camera(f, up, eye), u)
             b, _ = ctx.render_image(sv, t, abi.make_camera(f, up, eye), u)
             assert np.array_equal(a, b), abi.TRAVERSAL_NAMES[t]
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("tuning", [None, {"interleave": 2, "waves_per_simd": 8, "mix_classes": 1}])
+def test_mixed_views_batch_bit_exact(c3, tuning):
+    """64 views (dda / svo-df / esvo; W x H log-uniform in 256..1920 x
+    256..1080; orbit cameras at distance 0.5..3) over three resident volumes
+    in one batched launch: every frame equals the plain form's frame of the
+    same view, and a band of rows of one view per traversal equals the
+    oracle."""
+    ctx, grid, gv, nodes, st, sv = c3
+    scenes = {"lognormal1024": (grid, gv, [1024] * 3, nodes, sv, [1024] * 3, 1.0)}
+    owned = []
+    for name, g, em in (("shells128", synth.shells_128(), 4.0), ("ct512", synth.ct_body(), 1.6)):
+        v = ctx.grid_volume(g)
+        n, s2 = ctx.build_octree(v, tree=abi.TREE_ROPE)
+        t2 = ctx.svo_volume(n, s2["side"])
+        owned += [v, t2]
+        scenes[name] = (g, v, list(g.shape[2::-1]), n, t2, [s2["side"]] * 3, em)
+    spec = workloads.mixed_views(sorted(scenes))
+    views = []
+    for name, t, cam, W, H in spec:
+        g, v, gmd, n, t2, smd, em = scenes[name]
+        vol, md = (v, gmd) if t == abi.DDA else (t2, smd)
+        views.append((vol, t, abi.make_camera(*cam), abi.make_uniforms(W, H, md, emission=em), ctx.frame(W, H)))
+    b = ctx.batch()
+    if tuning:
+        tw = b.tuning()
+        b.set_tuning(tw["tile_w"], tw["tiles_per_grab"], tuning["waves_per_simd"], tw["class_kernels"],
+                     tuning["interleave"], tuning["mix_classes"])
+    b.set_views(views)
+    b.render()
+    checked = set()
+    for (name, t, cam, W, H), (vol, _, gcam, u, f) in zip(spec, views):
+        got = f.download()
+        ref, _ = ctx.render_image(vol, t, gcam, u)
+        assert np.array_equal(got, ref), (name, abi.TRAVERSAL_NAMES[t], W, H)
+        if t not in checked and got[..., :3].any():
+            checked.add(t)
+            g, v, gmd, n, t2, smd, em = scenes[name]
+            ov = xo.OracleVolume(grid=g) if t == abi.DDA else xo.OracleVolume(nodes=n, side=smd[0])
+            y = H // 2
+            ou = xo.make_uniforms(W, 2, gmd if t == abi.DDA else smd, emission=em, out_offset=(0, y),
+                                  disp_offset=(0, 0), disp_extent=(W, H))
+            oref = xo.render(t, ov, xo.make_camera(*cam), ou, threads=min(16, os.cpu_count() or 1))["rgba"]
+            assert np.array_equal(got[y:y + 2], oref), (name, abi.TRAVERSAL_NAMES[t], W, H)
+    assert checked == {abi.DDA, abi.SVO_DF, abi.ESVO}
+    b.free()
+    for v in views:
+        v[4].free()
+    for v in owned:
+        v.free()

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <initializer_list>
 #include <cstring>
 #include <vector>
 
@@ -406,8 +407,14 @@ int xd_batch_create(xd_context* ctx, xd_batch** out) {
     if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
     auto* b = new xd_batch();
     b->ctx = ctx;
+    // the fastest launch shape measured on the 1080p five-traversal batch:
+    // one kernel for every traversal, its waves starting on different
+    // traversals, 8 waves per SIMD, tiles in squares of 8
     for (int t = 0; t < 5; ++t) b->tuning.tile_w[t] = 8;
     b->tuning.tiles_per_grab = 1;
+    b->tuning.interleave = 2;
+    b->tuning.waves_per_simd = 8;
+    b->tuning.mix_classes = 1;
     if ((e = hipMalloc(&b->d_qbeg, sizeof(uint32_t) * (kClasses * kQueues + 1))) != hipSuccess ||
         (e = hipMalloc(&b->d_counters, sizeof(uint32_t) * kClasses * kQueues)) != hipSuccess) {
         xd_batch_free(b);
@@ -476,132 +483,77 @@ int xd_batch_render(xd_batch* b, double* ms, uint64_t* rays) {
     return run(b, ms);
 }
 
+namespace {
+// Best of `trials` timed launches of the batch's current tuning.
+int time_tuning(xd_batch* b, uint32_t trials, double* best) {
+    int rc = prepare(b);
+    if (rc) return rc;
+    *best = 1e30;
+    for (uint32_t k = 0; k < trials; ++k) {
+        double ms = 0;
+        if ((rc = run(b, &ms))) return rc;
+        *best = std::min(*best, ms);
+    }
+    return XD_OK;
+}
+
+// One coordinate of the search: times each candidate value of *field and
+// keeps the one already set unless another is at least 2 % faster (launch
+// times vary by about 1 % from run to run).
+int sweep(xd_batch* b, uint32_t trials, uint32_t* field, std::initializer_list<uint32_t> values) {
+    const uint32_t start = *field;
+    double cur = 0;
+    int rc = time_tuning(b, trials, &cur);
+    if (rc) return rc;
+    uint32_t pick = start;
+    double pick_ms = cur;
+    for (uint32_t v : values) {
+        if (v == start) continue;
+        *field = v;
+        double ms = 0;
+        if ((rc = time_tuning(b, trials, &ms))) return rc;
+        if (ms < 0.98 * cur && ms < pick_ms) {
+            pick = v;
+            pick_ms = ms;
+        }
+    }
+    *field = pick;
+    return XD_OK;
+}
+}  // namespace
+
 int xd_batch_autotune(xd_batch* b, uint32_t trials, xd_batch_tuning* chosen) {
     if (!b) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
     if (trials == 0) trials = 3;
     hipError_t e = hipSetDevice(b->ctx->device);
     if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
     const std::vector<xd_view> all = b->views;
-    xd_batch_tuning best = b->tuning;
+    const xd_batch_tuning start = b->tuning;
+    int rc = XD_OK;
     // tile shape per traversal, timed on that traversal's views alone
-    for (int t = 0; t < 5; ++t) {
+    for (int t = 0; t < 5 && !rc; ++t) {
         std::vector<xd_view> sub;
         for (const auto& v : all)
             if (v.traversal == t) sub.push_back(v);
         if (sub.empty()) continue;
-        double best_ms = 1e30;
-        for (uint32_t tw : {8u, 16u, 32u}) {
-            b->views = sub;
-            b->tuning.tile_w[t] = tw;
-            int rc = prepare(b);
-            if (rc) return rc;
-            double m = 1e30;
-            for (uint32_t k = 0; k < trials; ++k) {
-                double ms = 0;
-                if ((rc = run(b, &ms))) return rc;
-                m = std::min(m, ms);
-            }
-            if (m < best_ms) {
-                best_ms = m;
-                best.tile_w[t] = tw;
-            }
-        }
-        b->tuning.tile_w[t] = best.tile_w[t];
+        b->views = sub;
+        rc = sweep(b, trials, &b->tuning.tile_w[t], {8u, 16u, 32u});
     }
-    // tiles taken per queue grab, on the whole batch
+    // then the launch shape, on the whole batch, one field at a time from
+    // the starting tuning
     b->views = all;
-    double best_ms = 1e30;
-    for (uint32_t g : {1u, 2u, 4u}) {
-        b->tuning.tiles_per_grab = g;
-        int rc = prepare(b);
-        if (rc) return rc;
-        double m = 1e30;
-        for (uint32_t k = 0; k < trials; ++k) {
-            double ms = 0;
-            if ((rc = run(b, &ms))) return rc;
-            m = std::min(m, ms);
-        }
-        if (m < best_ms) {
-            best_ms = m;
-            best.tiles_per_grab = g;
-        }
+    if (!rc) rc = sweep(b, trials, &b->tuning.mix_classes, {0u, 1u});
+    if (!rc) rc = sweep(b, trials, &b->tuning.interleave, {0u, 1u, 2u, 3u, 4u, 5u});
+    if (!rc) rc = sweep(b, trials, &b->tuning.waves_per_simd, {0u, 8u});
+    if (!rc) rc = sweep(b, trials, &b->tuning.class_kernels, {0u, 1u});
+    if (!rc) rc = sweep(b, trials, &b->tuning.tiles_per_grab, {1u, 2u, 4u});
+    if (rc) {
+        b->tuning = start;
+        (void)prepare(b);
+        return rc;
     }
-    b->tuning.tiles_per_grab = best.tiles_per_grab;
-    // tile order: rows, interleaved across views of one group, or squares
-    best_ms = 1e30;
-    for (uint32_t k : {0u, 1u, 2u, 3u, 4u, 5u}) {
-        b->tuning.interleave = k;
-        int rc = prepare(b);
-        if (rc) return rc;
-        double m = 1e30;
-        for (uint32_t r = 0; r < trials; ++r) {
-            double ms = 0;
-            if ((rc = run(b, &ms))) return rc;
-            m = std::min(m, ms);
-        }
-        if (m < best_ms) {
-            best_ms = m;
-            best.interleave = k;
-        }
-    }
-    b->tuning.interleave = best.interleave;
-    // all-traversal kernel: waves on one class at a time, or a mix
-    best_ms = 1e30;
-    for (uint32_t k : {0u, 1u}) {
-        b->tuning.mix_classes = k;
-        int rc = prepare(b);
-        if (rc) return rc;
-        double m = 1e30;
-        for (uint32_t r = 0; r < trials; ++r) {
-            double ms = 0;
-            if ((rc = run(b, &ms))) return rc;
-            m = std::min(m, ms);
-        }
-        if (m < best_ms) {
-            best_ms = m;
-            best.mix_classes = k;
-        }
-    }
-    b->tuning.mix_classes = best.mix_classes;
-    // one kernel for every traversal, or one per traversal
-    best_ms = 1e30;
-    for (uint32_t k : {0u, 1u}) {
-        b->tuning.class_kernels = k;
-        int rc = prepare(b);
-        if (rc) return rc;
-        double m = 1e30;
-        for (uint32_t r = 0; r < trials; ++r) {
-            double ms = 0;
-            if ((rc = run(b, &ms))) return rc;
-            m = std::min(m, ms);
-        }
-        if (m < best_ms) {
-            best_ms = m;
-            best.class_kernels = k;
-        }
-    }
-    b->tuning.class_kernels = best.class_kernels;
-    // register budget: compiler's choice or 8 waves per SIMD
-    best_ms = 1e30;
-    for (uint32_t w : {0u, 8u}) {
-        b->tuning.waves_per_simd = w;
-        int rc = prepare(b);
-        if (rc) return rc;
-        double m = 1e30;
-        for (uint32_t k = 0; k < trials; ++k) {
-            double ms = 0;
-            if ((rc = run(b, &ms))) return rc;
-            m = std::min(m, ms);
-        }
-        if (m < best_ms) {
-            best_ms = m;
-            best.waves_per_simd = w;
-        }
-    }
-    b->tuning = best;
-    int rc = prepare(b);
-    if (rc) return rc;
-    if (chosen) *chosen = best;
+    if ((rc = prepare(b))) return rc;
+    if (chosen) *chosen = b->tuning;
     return XD_OK;
 }
 
