@@ -303,24 +303,42 @@ __attribute__((visibility("default"))) void xds_lognormal(void* out, uint64_t X,
     for (uint64_t i = 0; i < n3; ++i) lnrho[i] = (float)(sigma_ln * (re[i] - mean) / sd);
     std::vector<double>().swap(re);
     std::vector<double>().swap(im);
-    // trilinear upsampling (cell-centred, clamped at the faces)
-    auto sample = [&](uint64_t x, uint64_t y, uint64_t z) {
-        const double fx = std::min(std::max(((double)x + 0.5) * (double)n / (double)X - 0.5, 0.0), (double)(n - 1));
-        const double fy = std::min(std::max(((double)y + 0.5) * (double)n / (double)Y - 0.5, 0.0), (double)(n - 1));
-        const double fz = std::min(std::max(((double)z + 0.5) * (double)n / (double)Z - 0.5, 0.0), (double)(n - 1));
-        const uint64_t x0 = (uint64_t)fx, y0 = (uint64_t)fy, z0 = (uint64_t)fz;
-        const uint64_t x1 = std::min(x0 + 1, n - 1), y1 = std::min(y0 + 1, n - 1), z1 = std::min(z0 + 1, n - 1);
-        const double ax = fx - (double)x0, ay = fy - (double)y0, az = fz - (double)z0;
+    // trilinear upsampling of the log-density (cell-centred, clamped at the
+    // faces); the density is its exponential, so thresholds and the colour
+    // ramp work on the interpolated value directly.  Per-axis cells and
+    // weights are tabulated once.
+    struct Axis {
+        std::vector<uint64_t> i0, i1;
+        std::vector<double> a;
+    };
+    auto axis = [&](uint64_t m) {
+        Axis A;
+        A.i0.resize(m);
+        A.i1.resize(m);
+        A.a.resize(m);
+        for (uint64_t v = 0; v < m; ++v) {
+            const double f = std::min(std::max(((double)v + 0.5) * (double)n / (double)m - 0.5, 0.0), (double)(n - 1));
+            A.i0[v] = (uint64_t)f;
+            A.i1[v] = std::min(A.i0[v] + 1, n - 1);
+            A.a[v] = f - (double)A.i0[v];
+        }
+        return A;
+    };
+    const Axis ax_x = axis(X), ax_y = axis(Y), ax_z = axis(Z);
+    auto log_density = [&](uint64_t x, uint64_t y, uint64_t z) {
+        const uint64_t x0 = ax_x.i0[x], x1 = ax_x.i1[x], y0 = ax_y.i0[y], y1 = ax_y.i1[y];
+        const uint64_t z0 = ax_z.i0[z], z1 = ax_z.i1[z];
+        const double ax = ax_x.a[x], ay = ax_y.a[y], az = ax_z.a[z];
         auto at = [&](uint64_t i, uint64_t j, uint64_t k) { return (double)lnrho[(k * n + j) * n + i]; };
         const double c00 = at(x0, y0, z0) * (1 - ax) + at(x1, y0, z0) * ax;
         const double c10 = at(x0, y1, z0) * (1 - ax) + at(x1, y1, z0) * ax;
         const double c01 = at(x0, y0, z1) * (1 - ax) + at(x1, y0, z1) * ax;
         const double c11 = at(x0, y1, z1) * (1 - ax) + at(x1, y1, z1) * ax;
         const double c0 = c00 * (1 - ay) + c10 * ay, c1 = c01 * (1 - ay) + c11 * ay;
-        return std::exp(c0 * (1 - az) + c1 * az);
+        return c0 * (1 - az) + c1 * az;
     };
-    // the threshold: quantile of the upsampled density from a log histogram
-    // of every voxel (exact up to one bin of 1/4096 of the log range)
+    // the threshold: quantile of the upsampled log-density from a histogram
+    // of every voxel (exact up to one bin of 1/65536 of the log range)
     float lo = lnrho[0], hi = lnrho[0];
     for (float v : lnrho) {
         lo = std::min(lo, v);
@@ -337,7 +355,7 @@ __attribute__((visibility("default"))) void xds_lognormal(void* out, uint64_t X,
                 for (uint64_t z = t; z < Z; z += nt)
                     for (uint64_t y = 0; y < Y; ++y)
                         for (uint64_t x = 0; x < X; ++x) {
-                            const int b = (int)std::min<double>(bins - 1, std::max(0.0, (std::log(sample(x, y, z)) - blo) / bw));
+                            const int b = (int)std::min<double>(bins - 1, std::max(0.0, (log_density(x, y, z) - blo) / bw));
                             ++hist[t][b];
                         }
             });
@@ -357,7 +375,7 @@ __attribute__((visibility("default"))) void xds_lognormal(void* out, uint64_t X,
     parallel_z(Z, [&](uint64_t z) {
         for (uint64_t y = 0; y < Y; ++y)
             for (uint64_t x = 0; x < X; ++x) {
-                const double l = std::log(sample(x, y, z));
+                const double l = log_density(x, y, z);
                 uint32_t v = 0;
                 if (l >= ln_thr) {
                     const double t = std::min(1.0, (l - ln_thr) / (ln_max - ln_thr + 1e-12));
