@@ -25,6 +25,12 @@ constexpr uint64_t kMaxTaggedNodes = 1ull << 30;
 // at most 1 + 8 + 64 + 512 = 585 nodes (18.7 KB of child words)
 constexpr uint32_t kTopLevels = 4;
 constexpr uint32_t kMaxTopNodes = 585;
+// Start table of a tagged octree: for each cell of the level-kTableLevel grid
+// (64^3 cells) the node that find() reaches at that level, or the leaf that
+// ends the walk above it, with its level.  2 MiB per octree volume.
+constexpr uint32_t kTableLevel = 6;
+constexpr uint64_t kTableCells = 1ull << (3 * kTableLevel);
+constexpr uint64_t kTableBytes = kTableCells * 8;
 
 // Everything one traversal launch needs; passed by value as the kernel
 // argument (kernarg segment), so a launch needs no device-side setup.
@@ -57,6 +63,7 @@ struct RenderParams {
     uint32_t node_count;
     uint32_t top_nodes;  // child words of nodes [0, top_nodes) are staged in LDS
     uint32_t depth_exact;  // every internal node's depth field equals its depth in the tree
+    const uint2* table;    // start table (tagged layout), kTableCells entries {child word, level}
     uint32_t xcd_chunks;  // plain launches: block runs per XCD (0: hardware order)
     unsigned long long* words;  // word counter (instrumented variant only)
 };

@@ -93,6 +93,29 @@ void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint3
     for (auto& x : th) x.join();
 }
 
+namespace {
+void fill_table(const uint32_t* child_w, uint32_t word, uint32_t level, uint32_t x0, uint32_t y0, uint32_t z0,
+                uint2* table) {
+    const uint32_t n = 1u << (kTableLevel - level);  // cells per side below this node
+    if ((word & kTagLeaf) || level == kTableLevel) {
+        for (uint32_t z = z0; z < z0 + n; ++z)
+            for (uint32_t y = y0; y < y0 + n; ++y)
+                for (uint32_t x = x0; x < x0 + n; ++x)
+                    table[((uint64_t)z << (2 * kTableLevel)) | (y << kTableLevel) | x] = make_uint2(word, level);
+        return;
+    }
+    const uint64_t node = word & kIndexMask;
+    const uint32_t h = n / 2;
+    for (uint32_t k = 0; k < 8; ++k)  // child k = 4x + 2y + z
+        fill_table(child_w, child_w[node * 8 + k], level + 1, x0 + ((k >> 2) & 1u) * h, y0 + ((k >> 1) & 1u) * h,
+                   z0 + (k & 1u) * h, table);
+}
+}  // namespace
+
+void build_start_table(const uint32_t* child_w, uint32_t root_tag, uint2* table) {
+    fill_table(child_w, root_tag, 0, 0, 0, 0, table);
+}
+
 bool octree_depths_exact(const uint32_t* w, uint64_t n) {
     std::vector<uint8_t> depth(n, 0xFF);
     std::vector<uint32_t> level{0}, next;

@@ -103,9 +103,11 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
         p.nodes = vol->d_nodes;
         p.cw = vol->d_cw;
         p.meta = vol->d_meta;
+        p.table = vol->d_table;
         p.root_tag = vol->root_tag;
-        // levels 0..3 for the stackless traversals, 0..2 next to an LDS stack
-        p.top_nodes = vol->d_cw ? vol->top[(traversal == XD_SVO_NAIVE || traversal == XD_SVO_ROPE) ? 3 : 2] : 0;
+        // levels 0..3 for svo-rope, 0..2 next to an LDS stack; svo-naive starts
+        // every walk from the start table, below the top levels
+        p.top_nodes = !vol->d_cw || traversal == XD_SVO_NAIVE ? 0u : vol->top[traversal == XD_SVO_ROPE ? 3 : 2];
         p.stack_slots = vol->max_depth + 1;
         p.lds_slots = p.stack_slots;
         p.node_count = (uint32_t)vol->node_count;
@@ -140,7 +142,7 @@ uint64_t grid_footprint(const uint64_t dims[3]) {
 
 uint64_t svo_footprint(uint64_t node_count, bool raw) {
     if (raw || node_count > kMaxTaggedNodes) return node_count * kNodeWords * sizeof(uint32_t);
-    return node_count * 8 * sizeof(uint32_t) + node_count * sizeof(uint2);
+    return node_count * 8 * sizeof(uint32_t) + node_count * sizeof(uint2) + kTableBytes;
 }
 
 int check_view(const xd_uniforms* u) {
@@ -420,6 +422,14 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
         }
         const uint32_t root_ld = w[9], root_col = w[8];
         v->root_tag = (root_ld & kLeafMask) ? (kTagLeaf | ((root_col & 0xFFFFFFu) ? 0u : kTagEmpty)) : 0u;
+        std::vector<uint2> table(kTableCells);
+        build_start_table(cw.data(), v->root_tag, table.data());
+        if ((e = hipMalloc(&v->d_table, kTableBytes)) != hipSuccess ||
+            (e = hipMemcpy(v->d_table, table.data(), kTableBytes, hipMemcpyHostToDevice)) != hipSuccess) {
+            xd_volume_free(v);
+            return fail_hip(e, "octree upload");
+        }
+        v->bytes = svo_footprint(node_count, false);
     } else {
         if ((e = hipMalloc(&v->d_nodes, bytes)) != hipSuccess ||
             (e = hipMemcpy(v->d_nodes, nodes, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
@@ -477,6 +487,7 @@ int xd_volume_free(xd_volume* v) {
     if (v->d_nodes) (void)hipFree(v->d_nodes);
     if (v->d_cw) (void)hipFree(v->d_cw);
     if (v->d_meta) (void)hipFree(v->d_meta);
+    if (v->d_table) (void)hipFree(v->d_table);
     v->ctx->volumes.erase(v);
     delete v;
     return XD_OK;

@@ -46,6 +46,7 @@ struct xd_volume {
     uint32_t* d_nodes = nullptr;
     uint32_t* d_cw = nullptr;
     uint2* d_meta = nullptr;
+    uint2* d_table = nullptr;  // start table of the tagged layout (see build_start_table)
     uint32_t root_tag = 0;
     uint32_t depth_exact = 0;  // internal nodes' depth fields match their depth (see octree_depths_exact)
     uint32_t top[xd::kTopLevels] = {0, 0, 0, 0};  // nodes in breadth-first levels 0..L
@@ -81,5 +82,12 @@ void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint3
 // as in every tree the converter writes.  Traversals may then track depth
 // themselves instead of reading the field back.
 bool octree_depths_exact(const uint32_t* w, uint64_t n);
+// Start table of a tagged layout (child_w as build_tagged_layout writes it,
+// root_tag the root's own tags): entry (z << 12 | y << 6 | x) of the level-6
+// grid = {tagged word of the node find() reaches at level 6 for any point of
+// cell (x, y, z), or of the leaf that ends the walk above it; that node's
+// level}.  The root's word is 0 | root_tag.  The walk follows child words
+// for at most 6 levels, so any tree, cyclic ones included, gives a table.
+void build_start_table(const uint32_t* child_w, uint32_t root_tag, uint2* table);
 
 }  // namespace xd

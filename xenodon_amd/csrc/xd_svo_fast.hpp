@@ -25,6 +25,7 @@ struct FastSvo {
     uint32_t n_top;
     uint32_t n;            // node count (indices read back from a stack are clamped to it)
     uint32_t depth_exact;  // depth fields of internal nodes equal their tree depth
+    const uint2* table;    // start table: kTableCells entries {child word, level}, or null
 };
 
 // Child word k of node n: from LDS for the breadth-first top levels.
@@ -41,13 +42,44 @@ __device__ __forceinline__ uint32_t colour_if(const FastSvo& s, uint32_t node, u
     return (tag & kTagEmpty) ? 0u : s.meta[node].x;
 }
 
-// svo_naive.comp:8-27 with tagged pointers.  `tag` returns the found node's tags.
+// The child choices of find() at every level, packed: bit (22 - l) of
+// level_bits(p) is the comparison p >= offset + extent made at level l.  At
+// level l, offset + extent = (2m + 1) * 2^-(l+1) for the integer m that the
+// earlier choices fix, and offset is exactly that dyadic value (sums of
+// powers of two below 1 are exact in FP32 over the first 23 levels).  For p
+// in [0, 1) the comparison is therefore bit 0 of floor(p * 2^(l+1)), which is
+// bit (22 - l) of floor(p * 2^23); p < 0 (or NaN) fails every comparison and
+// p >= 1 passes every one.
+__device__ __forceinline__ uint32_t level_bits(float p) {
+    if (!(p > 0.0f)) return 0u;
+    if (!(p < 1.0f)) return (1u << 23) - 1u;
+    return (uint32_t)(p * 8388608.0f);
+}
+
+// svo_naive.comp:8-27 with tagged pointers.  `tag` returns the found node's
+// tags.  With a start table the walk begins at the node the table holds for
+// the point's level-6 cell (found from the point's level bits): the same node
+// the first six levels of the walk reach, with the same offset and extent,
+// which are exact dyadic values.
 __device__ __forceinline__ uint32_t fast_find(const FastSvo& s, f3 pos, f3& base, float& side, uint32_t& tag) {
     float extent = 1.0f;
     uint32_t index = 0;
+    uint32_t level = 0;
     f3 offset = splat(0.0f);
     tag = s.root_tag;
-    for (uint32_t level = 0;; ++level) {
+    if (s.table) {
+        const uint32_t bx = level_bits(pos.x), by = level_bits(pos.y), bz = level_bits(pos.z);
+        constexpr uint32_t sh = 23u - kTableLevel;
+        const uint2 e = s.table[((bz >> sh) << (2 * kTableLevel)) | ((by >> sh) << kTableLevel) | (bx >> sh)];
+        index = e.x & kIndexMask;
+        tag = e.x;
+        level = e.y;
+        extent = __builtin_ldexpf(1.0f, -(int)level);
+        const uint32_t c = 23u - level;  // 32-bit shifts: level >= 1 here, or the bits are unused
+        offset = level == 0 ? splat(0.0f)
+                            : smul(extent, mk((float)(bx >> c), (float)(by >> c), (float)(bz >> c)));
+    }
+    for (;; ++level) {
         if ((tag & kTagLeaf) || level > kMaxLevels) {
             base = offset;
             side = extent;

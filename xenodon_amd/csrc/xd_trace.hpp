@@ -159,7 +159,7 @@ __device__ __forceinline__ f3 svo_pixel(const RenderParams& p, f3 rd, uint32_t* 
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     uint32_t* stk_a = lds + wave * 2u * p.lds_slots * kWaveLanes + lane;
     uint32_t* stk_b = stk_a + p.lds_slots * kWaveLanes;
-    const FastSvo fs{p.cw, p.meta, p.root_tag, lds_top, p.top_nodes, p.node_count, p.depth_exact};
+    const FastSvo fs{p.cw, p.meta, p.root_tag, lds_top, p.top_nodes, p.node_count, p.depth_exact, p.table};
     if (TRAV == 2) {  // esvo
         float* stk_tmax = reinterpret_cast<float*>(stk_b);
         for (uint32_t s = 0; s < p.stack_slots; ++s) {
