@@ -21,6 +21,12 @@ constexpr uint32_t kTagLeaf = 0x80000000u;   // the child is a leaf
 constexpr uint32_t kTagEmpty = 0x40000000u;  // ... whose RGB is zero
 constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
 constexpr uint64_t kMaxTaggedNodes = 1ull << 30;
+// Trees of fewer than 2^25 nodes whose depth fields are below 32 keep, in
+// bits 25..29 of each leaf's child words (its ropes), the depth field of the
+// node the word points to, so svo-rope learns the neighbour's depth without
+// loading it.  Such a rope word's index is its low 25 bits.
+constexpr uint32_t kRopeDepthShift = 25;
+constexpr uint32_t kRopeIndexMask = (1u << kRopeDepthShift) - 1u;
 // top tree levels numbered breadth-first and staged in LDS: levels 0..3 hold
 // at most 1 + 8 + 64 + 512 = 585 nodes (18.7 KB of child words)
 constexpr uint32_t kTopLevels = 4;
@@ -64,6 +70,7 @@ struct RenderParams {
     uint32_t top_nodes;  // child words of nodes [0, top_nodes) are staged in LDS
     uint32_t depth_exact;  // every internal node's depth field equals its depth in the tree
     const uint2* table;    // start table (tagged layout), kTableCells entries {child word, level}
+    uint32_t rope_depth;   // leaves' child words carry their targets' depth fields (kRopeDepthShift)
     uint32_t xcd_chunks;  // plain launches: block runs per XCD (0: hardware order)
     unsigned long long* words;  // word counter (instrumented variant only)
 };

@@ -112,6 +112,20 @@ void fill_table(const uint32_t* child_w, uint32_t word, uint32_t level, uint32_t
 }
 }  // namespace
 
+bool pack_rope_depths(uint32_t* child_w, const uint32_t* meta, uint64_t n) {
+    if (n > (uint64_t)kRopeIndexMask + 1) return false;
+    for (uint64_t i = 0; i < n; ++i)
+        if ((meta[i * 2 + 1] & kDepthMask) > 31) return false;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (!(meta[i * 2 + 1] & kLeafMask)) continue;
+        for (int k = 0; k < 8; ++k) {
+            const uint64_t c = child_w[i * 8 + k] & kIndexMask;
+            child_w[i * 8 + k] |= (meta[c * 2 + 1] & kDepthMask) << kRopeDepthShift;
+        }
+    }
+    return true;
+}
+
 void build_start_table(const uint32_t* child_w, uint32_t root_tag, uint2* table) {
     fill_table(child_w, root_tag, 0, 0, 0, 0, table);
 }

@@ -104,6 +104,7 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
         p.cw = vol->d_cw;
         p.meta = vol->d_meta;
         p.table = vol->d_table;
+        p.rope_depth = vol->rope_depth;
         p.root_tag = vol->root_tag;
         // levels 0..3 for svo-rope, 0..2 next to an LDS stack; svo-naive starts
         // every walk from the start table, below the top levels
@@ -412,6 +413,7 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
         }
         build_tagged_layout(w, node_count, cw.data(), meta.data(), v->top);
         v->depth_exact = octree_depths_exact(w, node_count) ? 1u : 0u;
+        v->rope_depth = pack_rope_depths(cw.data(), meta.data(), node_count) ? 1u : 0u;
         if ((e = hipMalloc(&v->d_cw, node_count * 8 * sizeof(uint32_t))) != hipSuccess ||
             (e = hipMalloc(&v->d_meta, node_count * sizeof(uint2))) != hipSuccess ||
             (e = hipMemcpy(v->d_cw, cw.data(), node_count * 8 * sizeof(uint32_t), hipMemcpyHostToDevice)) !=

@@ -47,6 +47,7 @@ struct xd_volume {
     uint32_t* d_cw = nullptr;
     uint2* d_meta = nullptr;
     uint2* d_table = nullptr;  // start table of the tagged layout (see build_start_table)
+    uint32_t rope_depth = 0;   // ropes carry their targets' depth fields (see pack_rope_depths)
     uint32_t root_tag = 0;
     uint32_t depth_exact = 0;  // internal nodes' depth fields match their depth (see octree_depths_exact)
     uint32_t top[xd::kTopLevels] = {0, 0, 0, 0};  // nodes in breadth-first levels 0..L
@@ -89,5 +90,11 @@ bool octree_depths_exact(const uint32_t* w, uint64_t n);
 // level}.  The root's word is 0 | root_tag.  The walk follows child words
 // for at most 6 levels, so any tree, cyclic ones included, gives a table.
 void build_start_table(const uint32_t* child_w, uint32_t root_tag, uint2* table);
+// When the tree has fewer than 2^25 nodes and every depth field is below 32,
+// ORs into every child word of every leaf (its ropes) the depth field of the
+// node it points to, shifted by kRopeDepthShift, and returns true; else
+// changes nothing and returns false.  Internal nodes' child words keep their
+// 30-bit indices.
+bool pack_rope_depths(uint32_t* child_w, const uint32_t* meta, uint64_t n);
 
 }  // namespace xd

@@ -26,6 +26,7 @@ struct FastSvo {
     uint32_t n;            // node count (indices read back from a stack are clamped to it)
     uint32_t depth_exact;  // depth fields of internal nodes equal their tree depth
     const uint2* table;    // start table: kTableCells entries {child word, level}, or null
+    uint32_t rope_depth;   // ropes carry their targets' depth fields (kRopeDepthShift)
 };
 
 // Child word k of node n: from LDS for the breadth-first top levels.
@@ -307,7 +308,10 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
     return total;
 }
 
-// svo_rope.comp:65-135 (find_relative inlined)
+// svo_rope.comp:65-135 (find_relative inlined).  With s.rope_depth the depth
+// field of a rope's target comes with the rope word, so a hop needs no load
+// of the target's {color, depth} before its walk; the found leaf's colour is
+// loaded together with the next rope.
 __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f3 rd) {
     f3 sgn = mk(fsign(rd.x), fsign(rd.y), fsign(rd.z));
     const uint32_t nbx = 1u - (sgn.x > 0.0f ? 1u : 0u);
@@ -322,6 +326,7 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
     float t_max = min_elem(vmax(box_min, box_max));
     if (t_min > t_max) return splat(0.0f);
     t_min = gmax(t_min, 0.0f);
+    const uint32_t rope_mask = s.rope_depth ? kRopeIndexMask : kIndexMask;
     f3 pos = add(ro, smul(t_min, rd));
     f3 offset;
     float side;
@@ -337,7 +342,7 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
     f3 mask;
     uint32_t n = neighbor_index(nbx, nby, nbz, far, mask);
     uint32_t wd = cw_at(s, node, n);
-    node = wd & kIndexMask;
+    node = wd & rope_mask;
     tag = wd;
     offset = add(offset, muls(mul(mask, sgn), side));
     uint64_t it = 0;
@@ -345,9 +350,16 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
         if (++it > cap) break;
         pos = add(ro, smul(u_max, rd));
         // find_relative (svo_rope.comp:29-48) from the rope's target
-        const uint2 m = s.meta[node];
         const uint32_t start = node;
-        const int d = (int)(m.y & kDepthMask);
+        uint32_t start_col = 0;
+        int d;
+        if (s.rope_depth) {
+            d = (int)((wd >> kRopeDepthShift) & 31u);
+        } else {
+            const uint2 m = s.meta[node];
+            d = (int)(m.y & kDepthMask);
+            start_col = m.x;
+        }
         float extent = __builtin_ldexpf(1.0f, -d);
         const float inv = __builtin_ldexpf(1.0f, d);
         offset = sub(offset, mk(pmod(offset.x, extent, inv), pmod(offset.y, extent, inv), pmod(offset.z, extent, inv)));
@@ -370,11 +382,12 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
         u_min = max_elem(vmin(node_min, node_max));
         u_max = min_elem(far);
         step = u_max - gmax(u_min, 0.0f);
-        const uint32_t col = (tag & kTagEmpty) ? 0u : (node == start ? m.x : s.meta[node].x);
-        total = add(total, muls(unorm_rgb(col), step));
+        const uint32_t col =
+            (tag & kTagEmpty) ? 0u : (node == start && !s.rope_depth ? start_col : s.meta[node].x);
         n = neighbor_index(nbx, nby, nbz, far, mask);
-        wd = cw_at(s, node, n);
-        node = wd & kIndexMask;
+        wd = cw_at(s, node, n);  // in flight together with the colour
+        total = add(total, muls(unorm_rgb(col), step));
+        node = wd & rope_mask;
         tag = wd;
         offset = add(offset, muls(mul(mask, sgn), side));
     }

@@ -23,8 +23,9 @@ struct TaggedNodes {
     const uint32_t* child_w;  // node_count * 8 tagged child words
     const uint2* meta;        // node_count * {color, is_leaf_depth}
     uint32_t n;
+    uint32_t mask;  // kIndexMask, or kRopeIndexMask where ropes carry depths
     __device__ __forceinline__ uint32_t child(uint32_t n, uint32_t k) const {
-        return child_w[(uint64_t)n * 8 + k] & kIndexMask;
+        return child_w[(uint64_t)n * 8 + k] & mask;
     }
     __device__ __forceinline__ uint32_t color(uint32_t n) const { return meta[n].x; }
     __device__ __forceinline__ uint32_t leafdepth(uint32_t n) const { return meta[n].y; }
