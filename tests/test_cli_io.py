@@ -197,6 +197,9 @@ def test_cli_argument_errors(tmp_path):
         (["convert", "--chan-diff", "1", "--std-dev", "2", "a", "b"],
          "Error: --std-dev and --chan-diff are mutually exclusive"),
         (["convert", "missing.tif", "o.svo"], "Error reading 'missing.tif': Failed to open"),
+        (["convert", "--gpu", "x", "a", "b"], "Error: Invalid value for <gpu index> of parameter --gpu"),
+        (["convert", "--gpu", "0", "--dag", "a", "b"],
+         "Error: --gpu builds sparse and rope trees with the channel-difference heuristic only"),
         (["help", "a", "b"], "Error: Invalid usage of subcommand 'help'"),
         (["frobnicate"], "Error: Invalid subcommand 'frobnicate'"),
     ]

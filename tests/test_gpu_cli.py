@@ -117,3 +117,20 @@ def test_discard_output_and_errors(scene):
     r = subprocess.run([xio.CLI_PATH, "render", "--headless", "h.conf", "v.tif", "--camera", "bad_cams.txt"],
                        capture_output=True, text=True, cwd=d, timeout=300)
     assert "Error: Syntax error in camera input file" in r.stdout
+
+
+@pytest.mark.parametrize("opts,tree,kw", [([], xo.TREE_SPARSE, {}),
+                                          (["--rope", "--chan-diff", "20"], xo.TREE_ROPE, {"chan_diff": 20})])
+def test_convert_on_the_gpu_matches_oracle(scene, tmp_path, opts, tree, kw):
+    """`xenodon convert --gpu 0` builds on the device and writes the same
+    .svo as the reference converter's restatement."""
+    g = H.blocky_grid(48, 4, seed=23)
+    g[5:9, 3:7, 20:30, :3] = 0
+    src, dst = tmp_path / "g.tif", tmp_path / "g.svo"
+    xio.save_tiff(src, g)
+    r = subprocess.run([xio.CLI_PATH, "convert", "--gpu", "0", *opts, str(src), str(dst)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0 and "Error" not in r.stdout, r.stdout + r.stderr
+    nodes, side = xio.load_svo(dst)
+    ref, st = xo.build_octree(g, tree=tree, **kw)
+    assert side == st["side"] and np.array_equal(nodes, ref)

@@ -94,7 +94,11 @@ const char* kHelpConvert =
     "\n"
     "--std-dev <value>\n"
     "    Merge a region into one leaf when the standard deviation of its voxel\n"
-    "    colours around their average is at most <value>.\n";
+    "    colours around their average is at most <value>.\n"
+    "\n"
+    "--gpu <gpu-index>\n"
+    "    Build on that GPU instead of the host (sparse and rope trees with the\n"
+    "    channel-difference heuristic; the result is identical).\n";
 
 const char* kHelpRender =
     "Usage:\n"
@@ -183,9 +187,12 @@ void convert(const std::vector<std::string>& args) {
     bool dag = false, rope = false;
     int64_t chan_diff = -1;
     double stddev = -1;
+    int64_t gpu = -1;
     Command cmd;
     cmd.flags = {{&dag, "--dag"}, {&rope, "--rope"}};
     cmd.parameters = {
+        {[&](const std::string& a) { int64_t v; if (!parse_int(a, v) || v < 0 || v > 1024) return false; gpu = v; return true; },
+         "gpu index", "--gpu"},
         {[&](const std::string& a) { int64_t v; if (!parse_int(a, v) || v < 0 || v > 255) return false; chan_diff = v; return true; },
          "channel difference", "--chan-diff"},
         {[&](const std::string& a) { double v; if (!parse_float(a, v) || v < 0.0) return false; stddev = v; return true; },
@@ -204,6 +211,10 @@ void convert(const std::vector<std::string>& args) {
     }
     if (chan_diff >= 0 && stddev >= 0) {
         printf("Error: --std-dev and --chan-diff are mutually exclusive\n");
+        return;
+    }
+    if (gpu >= 0 && (dag || stddev >= 0)) {
+        printf("Error: --gpu builds sparse and rope trees with the channel-difference heuristic only\n");
         return;
     }
     printf("Loading source...\n");
@@ -225,8 +236,23 @@ void convert(const std::vector<std::string>& args) {
     xd_build_stats st{};
     const int heuristic = stddev >= 0 ? XD_HEURISTIC_STD_DEV : XD_HEURISTIC_CHAN_DIFF;
     const int type = dag ? XD_TREE_DAG : rope ? XD_TREE_ROPE : XD_TREE_SPARSE;
-    if (xd_build_octree(grid.texels.data(), grid.dim[0], grid.dim[1], grid.dim[2], heuristic,
-                        (int)std::max<int64_t>(chan_diff, 0), std::max(stddev, 0.0), type, &nodes, &st) != XD_OK) {
+    if (gpu >= 0) {
+        // the grid goes to HBM once; the tree comes back as .svo words
+        xd_context* ctx = nullptr;
+        xd_volume* vol = nullptr;
+        int rc = xd_open((int)gpu, &ctx);
+        if (rc == XD_OK) rc = xd_volume_from_grid(ctx, grid.texels.data(), grid.dim[0], grid.dim[1], grid.dim[2], &vol);
+        if (rc == XD_OK)
+            rc = xd_build_octree_device(ctx, vol, (int)std::max<int64_t>(chan_diff, 0), type, &nodes, &st);
+        const std::string msg = rc == XD_OK ? "" : xd_last_error();
+        xd_close(ctx);
+        if (rc != XD_OK) {
+            printf("Error: %s\n", msg.c_str());
+            return;
+        }
+    } else if (xd_build_octree(grid.texels.data(), grid.dim[0], grid.dim[1], grid.dim[2], heuristic,
+                               (int)std::max<int64_t>(chan_diff, 0), std::max(stddev, 0.0), type, &nodes, &st) !=
+               XD_OK) {
         printf("Error: %s\n", xd_last_error());
         return;
     }
