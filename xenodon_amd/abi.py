@@ -353,7 +353,8 @@ class Batch:
         return {"tile_w": list(t.tile_w), "tiles_per_grab": t.tiles_per_grab, "waves_per_simd": t.waves_per_simd,
                 "class_kernels": t.class_kernels, "interleave": t.interleave, "mix_classes": t.mix_classes}
 
-    def set_tuning(self, tile_w, tiles_per_grab=1, waves_per_simd=0, class_kernels=0, interleave=0, mix_classes=0):
+    def set_tuning(self, tile_w, tiles_per_grab=1, waves_per_simd=8, class_kernels=0, interleave=2, mix_classes=1):
+        """Defaults are the library's starting launch shape."""
         t = BatchTuning()
         t.tile_w[:] = list(tile_w)
         t.tiles_per_grab = tiles_per_grab

@@ -3,17 +3,18 @@
 //
 // Queue: every view is cut into wavefront tiles (8x8, 16x4 or 32x2 pixels,
 // chosen per traversal); a tile is one u32 (view << 20 | tile).  Tiles are
-// grouped by traversal class, one persistent kernel per class, so that each
-// kernel has the registers and LDS of its own traversal only (a kernel that
-// switches over all five is sized by the worst of them).  Within a class the
-// tiles are dealt into 8 ranges by the band of the view they lie in, one
-// range per XCD (the hardware deals workgroups to the XCDs round-robin, so
-// block b runs on XCD b % 8).  A wavefront takes tiles from its own XCD's
-// range with one atomic per grab, and when that range is exhausted it moves
-// on to the other ranges, so a kernel ends only when every tile of its class
-// is done and no CU idles while work is left.  Each tile is rendered by
-// exactly the per-ray code of the plain kernels, so every pixel is identical
-// to the plain form.
+// grouped by traversal class.  Either one persistent kernel drains every
+// class (by default, its waves starting on different classes so that each CU
+// runs latency-bound and issue-bound traversals side by side), or each class
+// gets a kernel with the registers and LDS of its own traversal only.  Within
+// a class the tiles are dealt into 8 ranges by the band of the view they lie
+// in, one range per XCD (the hardware deals workgroups to the XCDs
+// round-robin, so block b runs on XCD b % 8).  A wavefront takes tiles from
+// its own XCD's range with one atomic per grab, and when that range is
+// exhausted it moves on to the other ranges, so a kernel ends only when every
+// tile of its classes is done and no CU idles while work is left.  Each tile
+// is rendered by exactly the per-ray code of the plain kernels, so every
+// pixel is identical to the plain form.
 //
 // The launch (counter reset + the class kernels) is captured once into a
 // hipGraph and replayed for every batch; new cameras or views only rewrite

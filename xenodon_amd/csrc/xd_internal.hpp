@@ -30,7 +30,6 @@ constexpr uint32_t kRopeIndexMask = (1u << kRopeDepthShift) - 1u;
 // top tree levels numbered breadth-first and staged in LDS: levels 0..3 hold
 // at most 1 + 8 + 64 + 512 = 585 nodes (18.7 KB of child words)
 constexpr uint32_t kTopLevels = 4;
-constexpr uint32_t kMaxTopNodes = 585;
 // Start table of a tagged octree: for each cell of the level-kTableLevel grid
 // (64^3 cells) the node that find() reaches at that level, or the leaf that
 // ends the walk above it, with its level.  2 MiB per octree volume.
