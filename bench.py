@@ -331,6 +331,19 @@ def main():
     per_trav = {n: rays_per_frame * len(v) / (sum(v) * 1e3) for n, v in ms_acc.items()}
     mean_ms = {n: float(np.mean(v)) for n, v in ms_acc.items()}
 
+    # clamped pixels at this emission coefficient (untimed): worst of frames
+    # spread along the path
+    saturated = {}
+    for name in travs:
+        vol, md = vols[name]
+        worst = 0.0
+        for k in range(0, args.steps, max(1, args.steps // 10)):
+            f, up, eye = frame_of(k)
+            ctx.render(vol, abi.TRAVERSALS[name], abi.make_camera(f, up, eye), uni(md), frame)
+            img = frame.download()
+            worst = max(worst, float((img[..., :3] == 255).any(-1).mean()))
+        saturated[name] = round(worst, 5)
+
     # algorithmic bytes of every traversal for every frame (untimed)
     words_of = {}
     for name in travs:
@@ -499,7 +512,8 @@ def main():
         "data": "synthetic (seeded CT-like 512x361x512 RGBA8 grid + its lossless rope SVO; reference camera.txt)",
         "config": {"workload": "512x361x512 RGBA8 grid, all five traversals at 1920x1080, camera.txt path",
                    "form": f"batched: persistent ray-tile queue, {B} frames x {len(travs)} traversals per "
-                           "hipGraph launch", "emission_coeff": EMISSION, "traversals": travs,
+                           "hipGraph launch", "emission_coeff": EMISSION,
+                   "saturated_pixel_fraction": saturated, "traversals": travs,
                    "frames": len(groups) * B, "batch_tuning": tuning},
         "batched_vs_plain": round(value / plain_value, 3),
         "plain": {"value": round(plain_value, 2), "wall_s": round(plain_wall, 3),

@@ -116,6 +116,34 @@ def test_c1_shells_orbit_bit_exact(ctx, c1_scene, t):
         _assert_parity(img, ref, f"c1/{abi.TRAVERSAL_NAMES[t]}/frame{k}")
 
 
+def _saturated_fraction(img):
+    return float((img[..., :3] == 255).any(-1).mean())
+
+
+def test_c1_emission_saturates_at_most_one_percent(ctx, c1_scene):
+    """The 128^3 workload's emission coefficient (4.0) leaves at most 1 % of
+    the pixels of the orbit with a clamped channel, for every traversal."""
+    for t in TRAVERSALS:
+        _, gv, md = c1_scene.pick(t)
+        sat = [_saturated_fraction(ctx.render_image(gv, t, abi.make_camera(f, up, eye),
+                                                    abi.make_uniforms(256, 256, md, emission=C1_EMISSION))[0])
+               for f, up, eye in c1_orbit()]
+        print(f"{abi.TRAVERSAL_NAMES[t]}: saturated pixels {max(sat):.4%} (worst frame)")
+        assert max(sat) <= 0.01, (abi.TRAVERSAL_NAMES[t], sat)
+
+
+@pytest.mark.parametrize("t", TRAVERSALS)
+def test_saturating_emission_bit_exact(ctx, c1_scene, t):
+    """A saturating run (emission 40): the store's clamp to [0, 1] before the
+    unorm8 conversion matches the oracle when most of the volume's pixels
+    clamp."""
+    f, up, eye = c1_orbit()[1]
+    img, ref = _render_both(ctx, c1_scene, t, f, up, eye, 128, 128, emission=40.0)
+    _assert_parity(img, ref, f"saturating/{abi.TRAVERSAL_NAMES[t]}")
+    covered = float((img[..., :3] > 0).any(-1).mean())
+    assert _saturated_fraction(img) > 0.5 * covered > 0
+
+
 @pytest.mark.parametrize("t", TRAVERSALS)
 def test_camera_inside_volume(ctx, c1_scene, t):
     f, up, eye = H.look_at((0.52, 0.47, 0.45), (0.1, 0.9, 0.95))
