@@ -170,15 +170,28 @@ def mixed_views_leg(ctx, scenes, tuning, reps):
             "batched_vs_plain": round(batched / plain, 3), "batched_launch_ms": round(float(np.mean(batch_ms)), 3)}
 
 
-def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plain=False):
-    """Poisson arrivals at `rate` req/s for `duration_s` (or n_saturated
-    requests all released at once).  volumes: name -> (kind, data, model_dim).
-    Returns latency percentiles (ms) and aggregate Mrays/s; with plain, also
-    the plain form's rate on the same requests."""
-    srv = ctx.server(tuning=tuning)
+def load_served(srv, volumes):
+    """Admits every volume (name -> (kind, data, model_dim)) into the server;
+    returns name -> (volume id, kind)."""
     ids = {}
     for name, (kind, d, md) in volumes.items():
         ids[name] = (srv.load_grid(d) if kind == "grid" else srv.load_svo(*d)), kind
+    return ids
+
+
+def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plain=False, server=None):
+    """Poisson arrivals at `rate` req/s for `duration_s` (or n_saturated
+    requests all released at once).  volumes: name -> (kind, data, model_dim).
+    Returns latency percentiles (ms) and aggregate Mrays/s; with plain, also
+    the plain form's rate on the same requests.  server = (server, ids) from
+    load_served reuses resident volumes; otherwise a server is made for the
+    leg."""
+    if server is None:
+        srv = ctx.server(tuning=tuning)
+        ids = load_served(srv, volumes)
+    else:
+        srv, ids = server
+    st0 = srv.stats()
     n = n_saturated if n_saturated else int(rate * duration_s)
     named, rng = workloads.request_mix({k: v[1] for k, v in ids.items()}, n, seed)
     offsets = np.zeros(n) if n_saturated else np.cumsum(rng.exponential(1000.0 / rate, n))
@@ -201,7 +214,8 @@ def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plai
         done += srv.poll(512, timeout_ms=50)
     th.join()
     st = srv.stats()
-    srv.free()
+    if server is None:
+        srv.free()
     bad = [c for c in done if c["status"] != 0]
     if bad:
         raise RuntimeError(f"{len(bad)} served requests failed (status {bad[0]['status']})")
@@ -214,9 +228,10 @@ def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plai
            "p99_ms": round(float(np.percentile(lat, 99)), 3),
            "p50_device_ms": round(float(np.percentile(dev, 50)), 3),
            "p99_device_ms": round(float(np.percentile(dev, 99)), 3),
-           "mean_rays_per_request": round(rays / n), "launches": st["launches"],
-           "max_requests_per_launch": st["max_requests_per_launch"],
-           "device_busy_frac": round(st["busy_ms"] / wall, 3)}
+           "mean_rays_per_request": round(rays / n), "launches": st["launches"] - st0["launches"],
+           "device_busy_frac": round((st["busy_ms"] - st0["busy_ms"]) / wall, 3)}
+    if server is None:
+        res["max_requests_per_launch"] = st["max_requests_per_launch"]
     if plain:
         res["plain_same_requests"] = plain_same_requests(ctx, volumes, named)
         res["served_vs_plain"] = round(res["mrays_s"] / res["plain_same_requests"]["mrays_s"], 3)
@@ -267,6 +282,10 @@ def main():
     ap.add_argument("--serve-rates", default="200,1000,2000,3000", help="Poisson request rates (req/s); empty: skip")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-tuning", default="", help="served-form tuning fields as JSON, over the defaults")
+    ap.add_argument("--c5", action="store_true",
+                    help="also serve requests over the 2048^3 and 1024^3 log-normal volumes next to the others")
+    ap.add_argument("--c5-rates", default="50,200,500,1000,2000")
+    ap.add_argument("--c5-seconds", type=float, default=10.0)
     ap.add_argument("--serve-saturated", type=int, default=400,
                     help="requests released at once for the served form's peak rate (0: skip)")
     args = ap.parse_args()
@@ -448,6 +467,7 @@ def main():
 
     # ---------------- many views of three volumes in one batched launch ------
     mixed = None
+    c3_host = None
     if args.mixed_views_reps > 0:
         sg, ss = ctx.grid_volume(shells), ctx.svo_volume(snodes, sst["side"])
         scenes = {"shells128": (sg, [128] * 3, ss, [sst["side"]] * 3, 4.0),
@@ -459,6 +479,8 @@ def main():
             bg = ctx.grid_volume(big)
             bnodes, bst = ctx.build_octree(bg, tree=abi.TREE_ROPE)
             bs = ctx.svo_volume(bnodes, bst["side"])
+            if args.c5:
+                c3_host = (big, bnodes)
             del big, bnodes
             scenes["lognormal1024"] = (bg, [1024] * 3, bs, [bst["side"]] * 3, 1.0)
             extra = [bg, bs]
@@ -472,15 +494,15 @@ def main():
     # ---------------- served form: Poisson arrivals, continuous batching -----
     served = []
     rates = [float(r) for r in args.serve_rates.split(",") if r]
+    svols = {"ct512_grid": ("grid", grid, [512, 361, 512]), "ct512_svo": ("svo", (nodes, side), [side] * 3),
+             "shells128_grid": ("grid", shells, [128] * 3),
+             "shells128_svo": ("svo", (snodes, sst["side"]), [sst["side"]] * 3)}
+    # served steps are small and mixed: 8x8 tiles, one tile per grab, one
+    # kernel for every traversal per step
+    serve_tuning = {"tile_w": [8] * 5, "tiles_per_grab": 1, "waves_per_simd": 0, "class_kernels": 0}
+    if args.serve_tuning:
+        serve_tuning.update(json.loads(args.serve_tuning))
     if rates or args.serve_saturated:
-        svols = {"ct512_grid": ("grid", grid, [512, 361, 512]), "ct512_svo": ("svo", (nodes, side), [side] * 3),
-                 "shells128_grid": ("grid", shells, [128] * 3),
-                 "shells128_svo": ("svo", (snodes, sst["side"]), [sst["side"]] * 3)}
-        # served steps are small and mixed: 8x8 tiles, one tile per grab, one
-        # kernel for every traversal per step
-        serve_tuning = {"tile_w": [8] * 5, "tiles_per_grab": 1, "waves_per_simd": 0, "class_kernels": 0}
-        if args.serve_tuning:
-            serve_tuning.update(json.loads(args.serve_tuning))
         for k, r in enumerate(rates):
             res = served_leg(ctx, svols, r, args.serve_seconds, seed=1000 + k, tuning=serve_tuning)
             log(f"served @ {r:.0f} req/s: {res['mrays_s']} Mrays/s, p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms")
@@ -491,6 +513,43 @@ def main():
             log(f"served, {args.serve_saturated} requests at once: {res['mrays_s']} Mrays/s, p99 {res['p99_ms']} ms; "
                 f"plain on the same requests: {res['plain_same_requests']['mrays_s']} Mrays/s")
             served.append(res)
+
+    # ---------------- served form over the 2048^3 volumes (opt-in) ----------
+    served_c5 = None
+    if args.c5:
+        t0 = time.time()
+        if c3_host is None:
+            big = synth.lognormal(side=1024)
+            bg = ctx.grid_volume(big)
+            c3_host = (big, ctx.build_octree(bg, tree=abi.TREE_ROPE)[0])
+            bg.free()
+            del big
+        huge = synth.lognormal(side=2048)
+        hg = ctx.grid_volume(huge)
+        hnodes, hst = ctx.build_octree(hg, tree=abi.TREE_ROPE)
+        hg.free()
+        log(f"2048^3 grid + rope octree ({hnodes.shape[0]} nodes) built in {time.time() - t0:.1f}s")
+        c5vols = dict(svols)
+        c5vols.update({"lognormal1024_grid": ("grid", c3_host[0], [1024] * 3),
+                       "lognormal1024_svo": ("svo", (c3_host[1], 1024), [1024] * 3),
+                       "lognormal2048_grid": ("grid", huge, [2048] * 3),
+                       "lognormal2048_svo": ("svo", (hnodes, hst["side"]), [hst["side"]] * 3)})
+        srv = ctx.server(tuning=serve_tuning)
+        t0 = time.time()
+        ids = load_served(srv, c5vols)
+        log(f"served volumes resident ({srv.stats()['resident_bytes'] / 1e9:.1f} GB) in {time.time() - t0:.1f}s")
+        served_c5 = []
+        for k, r in enumerate(float(x) for x in args.c5_rates.split(",") if x):
+            res = served_leg(ctx, c5vols, r, args.c5_seconds, seed=2000 + k, tuning=serve_tuning, server=(srv, ids))
+            log(f"served (2048^3 resident) @ {r:.0f} req/s: {res['mrays_s']} Mrays/s, p99 {res['p99_ms']} ms")
+            served_c5.append(res)
+        res = served_leg(ctx, c5vols, 0, 0, seed=2999, tuning=serve_tuning, n_saturated=args.serve_saturated or 400,
+                         plain=True, server=(srv, ids))
+        log(f"served (2048^3 resident), requests at once: {res['mrays_s']} Mrays/s; plain "
+            f"{res['plain_same_requests']['mrays_s']} Mrays/s")
+        served_c5.append(res)
+        srv.free()
+        del huge, hnodes, c3_host
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -522,6 +581,7 @@ def main():
         "batched_per_traversal_mrays": {n: round(v, 1) for n, v in per_trav_batched.items()},
         "mixed_views": mixed,
         "served": served,
+        "served_2048": served_c5,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

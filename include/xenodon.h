@@ -117,9 +117,11 @@ XD_API int xd_close(xd_context* ctx);
 XD_API int xd_context_set_tile(xd_context* ctx, uint32_t tile_w);
 /* Named integer options of a context:
  *   "octree_layout"  0 (default) derive the tagged device layout for trees of
- *                    at most 2^30 nodes; 1 keep the .svo node words as they
- *                    are (the layout used above 2^30 nodes).  Applies to
- *                    volumes created afterwards; results are identical.
+ *                    at most 2^31 nodes (31-bit indices above 2^30); 1 keep
+ *                    the .svo node words as they are (the layout used above
+ *                    2^31 nodes); 2 the tagged layout with 31-bit indices
+ *                    whatever the size.  Applies to volumes created
+ *                    afterwards; results are identical.
  *   "xcd_chunks"     plain launches (xd_render): 0 (default) leaves the
  *                    image's pixel blocks in dispatch order, so neighbouring
  *                    blocks run on different XCDs; N in [1, 64] cuts the

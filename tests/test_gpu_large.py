@@ -71,7 +71,7 @@ def test_c3_esvo_on_svo_above_4gb_bit_exact(c3):
 
 @pytest.mark.timeout(600)
 def test_c3_raw_octree_words_with_64bit_offsets(c3):
-    """The .svo words kept as they are (the layout above 2^30 nodes): one
+    """The .svo words kept as they are (the layout above 2^31 nodes): one
     6.7 GB buffer addressed with 64-bit offsets; frames equal the tagged
     layout's."""
     ctx, grid, gv, nodes, st, sv = c3

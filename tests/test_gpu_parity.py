@@ -245,12 +245,15 @@ def test_malformed_octree_is_rejected(ctx):
     assert "cycle" in str(e.value)
 
 
+@pytest.mark.parametrize("layout", [1, 2])
 @pytest.mark.parametrize("t", [abi.SVO_NAIVE, abi.ESVO, abi.SVO_DF, abi.SVO_ROPE])
-def test_raw_octree_layout_bit_exact(t):
-    """Trees above 2^30 nodes keep the .svo node words on the device; the
-    context option forces that layout so its kernels are checked here."""
+def test_raw_and_wide_octree_layouts_bit_exact(t, layout):
+    """Trees above 2^31 nodes keep the .svo node words on the device (layout
+    1), trees of 2^30..2^31 nodes get 31-bit child indices without the empty
+    tag (layout 2); the context option forces either layout on a small tree
+    so their kernels are checked here."""
     with abi.Context(0) as ctx:
-        ctx.set_option("octree_layout", 1)
+        ctx.set_option("octree_layout", layout)
         grid = synth.shells_128()
         nodes, st = xo.build_octree(grid, tree=xo.TREE_ROPE)
         gv = ctx.svo_volume(nodes, st["side"])
@@ -259,7 +262,7 @@ def test_raw_octree_layout_bit_exact(t):
             u_args = (200, 150, [st["side"]] * 3)
             img, _ = ctx.render_image(gv, t, abi.make_camera(f, up, eye), abi.make_uniforms(*u_args, emission=4.0))
             ref = xo.render(t, ov, xo.make_camera(f, up, eye), xo.make_uniforms(*u_args, emission=4.0))
-            _assert_parity(img, ref["rgba"], f"raw/{abi.TRAVERSAL_NAMES[t]}/frame{k}")
+            _assert_parity(img, ref["rgba"], f"layout{layout}/{abi.TRAVERSAL_NAMES[t]}/frame{k}")
             words = ctx.count_words(gv, t, abi.make_camera(f, up, eye), abi.make_uniforms(*u_args, emission=4.0))
             ref_w = xo.render(t, ov, xo.make_camera(f, up, eye), xo.make_uniforms(*u_args, emission=4.0),
                               words=True)["words"].sum()

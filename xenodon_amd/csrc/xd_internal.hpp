@@ -20,7 +20,12 @@ constexpr uint32_t kWaveLanes = 64;
 constexpr uint32_t kTagLeaf = 0x80000000u;   // the child is a leaf
 constexpr uint32_t kTagEmpty = 0x40000000u;  // ... whose RGB is zero
 constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
-constexpr uint64_t kMaxTaggedNodes = 1ull << 30;
+// Tagged child words hold a 30-bit index with both tags.  Trees of 2^30 to
+// 2^31 nodes use wide tags: a 31-bit index and the leaf tag only (no empty
+// tag; such leaves load their colour, which is 0).
+constexpr uint64_t kMaxNarrowTagNodes = 1ull << 30;
+constexpr uint64_t kMaxTaggedNodes = 1ull << 31;
+constexpr uint32_t kWideIndexMask = 0x7FFFFFFFu;
 // Trees of fewer than 2^25 nodes whose depth fields are below 32 keep, in
 // bits 25..29 of each leaf's child words (its ropes), the depth field of the
 // node the word points to, so svo-rope learns the neighbour's depth without
@@ -70,6 +75,8 @@ struct RenderParams {
     uint32_t depth_exact;  // every internal node's depth field equals its depth in the tree
     const uint2* table;    // start table (tagged layout), kTableCells entries {child word, level}
     uint32_t rope_depth;   // leaves' child words carry their targets' depth fields (kRopeDepthShift)
+    uint32_t index_mask;   // index bits of a tagged child word (kIndexMask, kWideIndexMask, kRopeIndexMask)
+    uint32_t empty_tag;    // kTagEmpty, or 0 for wide tags
     uint32_t xcd_chunks;  // plain launches: block runs per XCD (0: hardware order)
     unsigned long long* words;  // word counter (instrumented variant only)
 };

@@ -25,7 +25,9 @@
 
 namespace xd {
 
-void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels]) {
+void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels],
+                         bool wide) {
+    const uint32_t empty_tag = wide ? 0u : kTagEmpty;
     std::vector<uint32_t> new_id(n, 0xFFFFFFFFu);
     uint32_t next = 1;
     new_id[0] = 0;
@@ -83,7 +85,7 @@ void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint3
                     const uint32_t c = nd[k];
                     const uint32_t* cn = w + (uint64_t)c * kNodeWords;
                     uint32_t tag = 0;
-                    if (cn[9] & kLeafMask) tag = kTagLeaf | ((cn[8] & 0xFFFFFFu) ? 0u : kTagEmpty);
+                    if (cn[9] & kLeafMask) tag = kTagLeaf | ((cn[8] & 0xFFFFFFu) ? 0u : empty_tag);
                     child_w[(uint64_t)ni * 8 + k] = new_id[c] | tag;
                 }
                 meta[(uint64_t)ni * 2] = nd[8];
@@ -94,8 +96,8 @@ void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint3
 }
 
 namespace {
-void fill_table(const uint32_t* child_w, uint32_t word, uint32_t level, uint32_t x0, uint32_t y0, uint32_t z0,
-                uint2* table) {
+void fill_table(const uint32_t* child_w, uint32_t mask, uint32_t word, uint32_t level, uint32_t x0, uint32_t y0,
+                uint32_t z0, uint2* table) {
     const uint32_t n = 1u << (kTableLevel - level);  // cells per side below this node
     if ((word & kTagLeaf) || level == kTableLevel) {
         for (uint32_t z = z0; z < z0 + n; ++z)
@@ -104,10 +106,10 @@ void fill_table(const uint32_t* child_w, uint32_t word, uint32_t level, uint32_t
                     table[((uint64_t)z << (2 * kTableLevel)) | (y << kTableLevel) | x] = make_uint2(word, level);
         return;
     }
-    const uint64_t node = word & kIndexMask;
+    const uint64_t node = word & mask;
     const uint32_t h = n / 2;
     for (uint32_t k = 0; k < 8; ++k)  // child k = 4x + 2y + z
-        fill_table(child_w, child_w[node * 8 + k], level + 1, x0 + ((k >> 2) & 1u) * h, y0 + ((k >> 1) & 1u) * h,
+        fill_table(child_w, mask, child_w[node * 8 + k], level + 1, x0 + ((k >> 2) & 1u) * h, y0 + ((k >> 1) & 1u) * h,
                    z0 + (k & 1u) * h, table);
 }
 }  // namespace
@@ -126,8 +128,8 @@ bool pack_rope_depths(uint32_t* child_w, const uint32_t* meta, uint64_t n) {
     return true;
 }
 
-void build_start_table(const uint32_t* child_w, uint32_t root_tag, uint2* table) {
-    fill_table(child_w, root_tag, 0, 0, 0, 0, table);
+void build_start_table(const uint32_t* child_w, uint32_t root_tag, uint32_t index_mask, uint2* table) {
+    fill_table(child_w, index_mask, root_tag, 0, 0, 0, 0, table);
 }
 
 bool octree_depths_exact(const uint32_t* w, uint64_t n) {

@@ -105,6 +105,8 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
         p.meta = vol->d_meta;
         p.table = vol->d_table;
         p.rope_depth = vol->rope_depth;
+        p.index_mask = vol->index_mask;
+        p.empty_tag = vol->empty_tag;
         p.root_tag = vol->root_tag;
         // levels 0..3 for svo-rope, 0..2 next to an LDS stack; svo-naive starts
         // every walk from the start table, below the top levels
@@ -225,8 +227,8 @@ int xd_context_set_tile(xd_context* ctx, uint32_t tile_w) {
 int xd_context_set_option(xd_context* ctx, const char* name, int64_t value) {
     if (!ctx || !name) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
     if (strcmp(name, "octree_layout") == 0) {
-        if (value != 0 && value != 1) return fail(XD_ERR_INVALID_ARGUMENT, "octree_layout must be 0 or 1");
-        ctx->raw_octrees = (int)value;
+        if (value < 0 || value > 2) return fail(XD_ERR_INVALID_ARGUMENT, "octree_layout must be 0, 1 or 2");
+        ctx->octree_layout = (int)value;
         return XD_OK;
     }
     if (strcmp(name, "tile_w") == 0) return xd_context_set_tile(ctx, (uint32_t)value);
@@ -401,7 +403,7 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
     v->is_rope = is_rope;
     const uint64_t bytes = node_count * kNodeWords * sizeof(uint32_t);
     v->bytes = bytes;
-    if (node_count <= kMaxTaggedNodes && !ctx->raw_octrees) {
+    if (node_count <= kMaxTaggedNodes && ctx->octree_layout != 1) {
         // tagged, sibling-grouped layout, derived on the host
         std::vector<uint32_t> cw, meta;
         try {
@@ -411,9 +413,12 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
             xd_volume_free(v);
             return fail(XD_ERR_OUT_OF_MEMORY, "out of host memory deriving the octree layout");
         }
-        build_tagged_layout(w, node_count, cw.data(), meta.data(), v->top);
+        const bool wide = node_count > kMaxNarrowTagNodes || ctx->octree_layout == 2;
+        build_tagged_layout(w, node_count, cw.data(), meta.data(), v->top, wide);
         v->depth_exact = octree_depths_exact(w, node_count) ? 1u : 0u;
-        v->rope_depth = pack_rope_depths(cw.data(), meta.data(), node_count) ? 1u : 0u;
+        v->rope_depth = !wide && pack_rope_depths(cw.data(), meta.data(), node_count) ? 1u : 0u;
+        v->index_mask = v->rope_depth ? kRopeIndexMask : wide ? kWideIndexMask : kIndexMask;
+        v->empty_tag = wide ? 0u : kTagEmpty;
         if ((e = hipMalloc(&v->d_cw, node_count * 8 * sizeof(uint32_t))) != hipSuccess ||
             (e = hipMalloc(&v->d_meta, node_count * sizeof(uint2))) != hipSuccess ||
             (e = hipMemcpy(v->d_cw, cw.data(), node_count * 8 * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
@@ -423,9 +428,9 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
             return fail_hip(e, "octree upload");
         }
         const uint32_t root_ld = w[9], root_col = w[8];
-        v->root_tag = (root_ld & kLeafMask) ? (kTagLeaf | ((root_col & 0xFFFFFFu) ? 0u : kTagEmpty)) : 0u;
+        v->root_tag = (root_ld & kLeafMask) ? (kTagLeaf | ((root_col & 0xFFFFFFu) ? 0u : v->empty_tag)) : 0u;
         std::vector<uint2> table(kTableCells);
-        build_start_table(cw.data(), v->root_tag, table.data());
+        build_start_table(cw.data(), v->root_tag, v->index_mask, table.data());
         if ((e = hipMalloc(&v->d_table, kTableBytes)) != hipSuccess ||
             (e = hipMemcpy(v->d_table, table.data(), kTableBytes, hipMemcpyHostToDevice)) != hipSuccess) {
             xd_volume_free(v);
@@ -458,7 +463,7 @@ int xd_volume_footprint(const xd_context* ctx, int kind, uint64_t dim_x, uint64_
     } else if (kind == XD_VOLUME_SVO) {
         if (node_count == 0 || node_count > 0xFFFFFFFFull)
             return fail(XD_ERR_FORMAT, "octree node count %llu is not in 1..2^32-1", (unsigned long long)node_count);
-        *bytes = svo_footprint(node_count, ctx && ctx->raw_octrees);
+        *bytes = svo_footprint(node_count, ctx && ctx->octree_layout == 1);
     } else {
         return fail(XD_ERR_INVALID_ARGUMENT, "unknown volume kind %d", kind);
     }

@@ -12,7 +12,7 @@
 struct xd_context {
     int device = 0;
     int tile_w = 8;  // per-wavefront tile width: 8 (8x8), 16 (16x4), 32 (32x2)
-    int raw_octrees = 0;  // keep .svo words instead of the tagged layout
+    int octree_layout = 0;  // 0 tagged (wide tags above 2^30 nodes), 1 .svo words, 2 wide tags always
     uint32_t xcd_chunks = 0;  // plain launches: block runs per XCD (see block_pixel)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -42,12 +42,14 @@ struct xd_volume {
     uint32_t macro_shift = 0;
     uint32_t macro_words = 0;
     // octree: the tagged layout (child words + {color, depth}) when the tree
-    // has at most 2^30 nodes, else the .svo words as they are
+    // has at most 2^31 nodes, else the .svo words as they are
     uint32_t* d_nodes = nullptr;
     uint32_t* d_cw = nullptr;
     uint2* d_meta = nullptr;
     uint2* d_table = nullptr;  // start table of the tagged layout (see build_start_table)
     uint32_t rope_depth = 0;   // ropes carry their targets' depth fields (see pack_rope_depths)
+    uint32_t index_mask = 0;   // index bits of the tagged child words
+    uint32_t empty_tag = 0;    // kTagEmpty, or 0 for wide tags
     uint32_t root_tag = 0;
     uint32_t depth_exact = 0;  // internal nodes' depth fields match their depth (see octree_depths_exact)
     uint32_t top[xd::kTopLevels] = {0, 0, 0, 0};  // nodes in breadth-first levels 0..L
@@ -77,7 +79,9 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
 // Host-side derivation of the tagged, sibling-grouped device layout
 // (child_w: n*8 words, meta: n*2 words) from .svo node words.
 // top[L] = number of nodes in levels 0..L, which hold indices [0, top[L]).
-void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels]);
+// wide: 31-bit indices and the leaf tag only (trees above 2^30 nodes).
+void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels],
+                         bool wide);
 // True when every internal node reachable from the root is reached only at
 // one depth and its is_leaf_depth depth field equals that depth (root = 0),
 // as in every tree the converter writes.  Traversals may then track depth
@@ -89,7 +93,7 @@ bool octree_depths_exact(const uint32_t* w, uint64_t n);
 // cell (x, y, z), or of the leaf that ends the walk above it; that node's
 // level}.  The root's word is 0 | root_tag.  The walk follows child words
 // for at most 6 levels, so any tree, cyclic ones included, gives a table.
-void build_start_table(const uint32_t* child_w, uint32_t root_tag, uint2* table);
+void build_start_table(const uint32_t* child_w, uint32_t root_tag, uint32_t index_mask, uint2* table);
 // When the tree has fewer than 2^25 nodes and every depth field is below 32,
 // ORs into every child word of every leaf (its ropes) the depth field of the
 // node it points to, shifted by kRopeDepthShift, and returns true; else

@@ -3,6 +3,8 @@
 // Layout: child_w[n*8 + k] = index of child k | kTagLeaf if that child is a
 // leaf | kTagEmpty if it is a leaf whose RGB is zero; meta[n] = {color,
 // is_leaf_depth}.  A leaf's child words are its ropes, tagged the same way.
+// Trees above 2^30 nodes carry 31-bit indices and no empty tag (index_mask
+// and empty_tag of FastSvo say which).
 // The traversal arithmetic is the shaders' (see xd_svo_ref.hpp); only loads
 // change:
 //   * "is this node a leaf?" comes with the pointer that led to it, so a
@@ -27,6 +29,8 @@ struct FastSvo {
     uint32_t depth_exact;  // depth fields of internal nodes equal their tree depth
     const uint2* table;    // start table: kTableCells entries {child word, level}, or null
     uint32_t rope_depth;   // ropes carry their targets' depth fields (kRopeDepthShift)
+    uint32_t index_mask;   // index bits of a child word
+    uint32_t empty_tag;    // kTagEmpty, or 0 where leaves carry no empty tag
 };
 
 // Child word k of node n: from LDS for the breadth-first top levels.
@@ -40,7 +44,7 @@ __device__ __forceinline__ uint32_t cw_at(const FastSvo& s, uint32_t n, uint32_t
 __device__ __forceinline__ float pmod(float x, float y, float inv_y) { return x - y * __builtin_floorf(x * inv_y); }
 
 __device__ __forceinline__ uint32_t colour_if(const FastSvo& s, uint32_t node, uint32_t tag) {
-    return (tag & kTagEmpty) ? 0u : s.meta[node].x;
+    return (tag & s.empty_tag) ? 0u : s.meta[node].x;
 }
 
 // The child choices of find() at every level, packed: bit (22 - l) of
@@ -72,7 +76,7 @@ __device__ __forceinline__ uint32_t fast_find(const FastSvo& s, f3 pos, f3& base
         const uint32_t bx = level_bits(pos.x), by = level_bits(pos.y), bz = level_bits(pos.z);
         constexpr uint32_t sh = 23u - kTableLevel;
         const uint2 e = s.table[((bz >> sh) << (2 * kTableLevel)) | ((by >> sh) << kTableLevel) | (bx >> sh)];
-        index = e.x & kIndexMask;
+        index = e.x & s.index_mask;
         tag = e.x;
         level = e.y;
         extent = __builtin_ldexpf(1.0f, -(int)level);
@@ -93,7 +97,7 @@ __device__ __forceinline__ uint32_t fast_find(const FastSvo& s, f3 pos, f3& base
         uint32_t child = (mx ? 4u : 0u) + (my ? 2u : 0u) + (mz ? 1u : 0u);
         offset = add(offset, mul(mk(mx ? 1.0f : 0.0f, my ? 1.0f : 0.0f, mz ? 1.0f : 0.0f), splat(extent)));
         const uint32_t wd = cw_at(s, index, child);
-        index = wd & kIndexMask;
+        index = wd & s.index_mask;
         tag = wd;
     }
 }
@@ -257,10 +261,10 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
             const uint32_t wd = pick8(lo, hi, ci);
             const bool bx = (ci & 4u) != 0, by = (ci & 2u) != 0, bz = (ci & 1u) != 0;
             if (wd & kTagLeaf) {
-                if (!(wd & kTagEmpty)) {  // an empty leaf adds exactly +0
+                if (!(wd & s.empty_tag)) {  // an empty leaf adds exactly +0
                     float t_min, t_max;
                     child_trange(corner, side, rrd, bias, ci, t_min, t_max);
-                    total = add(total, muls(unorm_rgb(s.meta[wd & kIndexMask].x), t_max - gmax(t_min, 0.0f)));
+                    total = add(total, muls(unorm_rgb(s.meta[wd & s.index_mask].x), t_max - gmax(t_min, 0.0f)));
                 }
                 if (mask != 0) continue;  // next sibling, from registers
             } else if (!(push = s.depth_exact ? mask != 0 : ci != 7u) || sp < cap_sp) {
@@ -272,7 +276,7 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
                 corner = mk(bx ? corner.x + side : corner.x, by ? corner.y + side : corner.y,
                             bz ? corner.z + side : corner.z);
                 side = side * 0.5f;
-                node = wd & kIndexMask;
+                node = wd & s.index_mask;
                 depth = depth + 1u;
                 pop = false;
             } else if (mask != 0) {
@@ -326,7 +330,7 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
     float t_max = min_elem(vmax(box_min, box_max));
     if (t_min > t_max) return splat(0.0f);
     t_min = gmax(t_min, 0.0f);
-    const uint32_t rope_mask = s.rope_depth ? kRopeIndexMask : kIndexMask;
+    const uint32_t rope_mask = s.index_mask;  // kRopeIndexMask when ropes carry depths
     f3 pos = add(ro, smul(t_min, rd));
     f3 offset;
     float side;
@@ -372,7 +376,7 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
             uint32_t child = (mx ? 4u : 0u) + (my ? 2u : 0u) + (mz ? 1u : 0u);
             offset = add(offset, mul(mk(mx ? 1.0f : 0.0f, my ? 1.0f : 0.0f, mz ? 1.0f : 0.0f), splat(extent)));
             wd = cw_at(s, node, child);
-            node = wd & kIndexMask;
+            node = wd & s.index_mask;
             tag = wd;
         }
         side = extent;
@@ -383,7 +387,7 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
         u_max = min_elem(far);
         step = u_max - gmax(u_min, 0.0f);
         const uint32_t col =
-            (tag & kTagEmpty) ? 0u : (node == start && !s.rope_depth ? start_col : s.meta[node].x);
+            (tag & s.empty_tag) ? 0u : (node == start && !s.rope_depth ? start_col : s.meta[node].x);
         n = neighbor_index(nbx, nby, nbz, far, mask);
         wd = cw_at(s, node, n);  // in flight together with the colour
         total = add(total, muls(unorm_rgb(col), step));
@@ -414,7 +418,7 @@ __device__ __forceinline__ f3 fast_esvo(const FastSvo& s, uint64_t cap, uint32_t
             float tv_max = gmin(t_max, tc_max);
             if (t_min <= tv_max) {
                 const uint32_t wd = cw_at(s, parent, idx ^ r.octant_mask);
-                const uint32_t child = wd & kIndexMask;
+                const uint32_t child = wd & s.index_mask;
                 if (wd & kTagLeaf) {
                     const uint32_t col = colour_if(s, child, wd);
                     total = add(total, muls(unorm_rgb(pend_col), pend_w));  // +0 the first time

@@ -23,7 +23,7 @@ struct TaggedNodes {
     const uint32_t* child_w;  // node_count * 8 tagged child words
     const uint2* meta;        // node_count * {color, is_leaf_depth}
     uint32_t n;
-    uint32_t mask;  // kIndexMask, or kRopeIndexMask where ropes carry depths
+    uint32_t mask;  // index bits of a child word (the volume's index_mask)
     __device__ __forceinline__ uint32_t child(uint32_t n, uint32_t k) const {
         return child_w[(uint64_t)n * 8 + k] & mask;
     }
