@@ -452,14 +452,25 @@ def main():
         sub.free()
 
     traffic = None
+    measured = {}
     if os.path.exists(PROFILE_SUMMARY):
+        # HBM bytes per launch of the same launch shape from the committed
+        # rocprofv3 counter runs; over this run's launch time they give the
+        # physically bounded rate next to the algorithmic one
         prof = json.load(open(PROFILE_SUMMARY))
         traffic = prof.get("hbm_bytes_per_launch", {}).get("batch")
+        k = prof.get("per_kernel", {}).get("batch", {})
+        if traffic:
+            measured = {"measured_hbm_GBs": round(traffic / (mean_batch_ms * 1e-3) / 1e9, 1),
+                        "measured_frac": round(traffic / (mean_batch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "l2_hit": k.get("l2_hit"), "valu_lane_utilisation": k.get("valu_lane_utilisation")}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "xd::k_batch (persistent, all five traversals)",
+                "kernel": "xd::" + ("k_batch_w8" if tuning["waves_per_simd"] == 8 else "k_batch")
+                          + ("<0..4> (persistent, one per traversal)" if tuning["class_kernels"]
+                             else "<5> (persistent, all five traversals)"),
                 "algorithmic_bytes_per_launch": round(float(np.mean(batch_algo))),
-                "mean_launch_ms": round(mean_batch_ms, 4),
+                "mean_launch_ms": round(mean_batch_ms, 4), **measured,
                 "plain_per_traversal_algorithmic_GBs": plain_roof}
 
     shells = synth.shells_128()
