@@ -16,6 +16,7 @@
 // leaves' child words are their ropes, renumbered and tagged the same way.
 // meta[i] = {color, is_leaf_depth} of node i.
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <thread>
@@ -24,9 +25,89 @@
 #include "xd_runtime.hpp"
 
 namespace xd {
+namespace {
+
+unsigned worker_count() { return std::max(1u, std::min(16u, std::thread::hardware_concurrency())); }
+
+}  // namespace
+
+bool octree_is_tree(const uint32_t* w, uint64_t n) {
+    std::vector<std::atomic<uint64_t>> seen((n + 63) / 64);
+    for (auto& x : seen) x.store(0, std::memory_order_relaxed);
+    std::atomic<bool> ok{true};
+    const unsigned T = worker_count();
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; ++t)
+        th.emplace_back([&, t] {
+            for (uint64_t i = t; i < n && ok.load(std::memory_order_relaxed); i += T) {
+                const uint32_t* nd = w + i * kNodeWords;
+                if (nd[9] & kLeafMask) continue;
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t c = nd[k];
+                    const uint64_t bit = 1ull << (c & 63u);
+                    if (c == 0 || (seen[c >> 6].fetch_or(bit, std::memory_order_relaxed) & bit)) {
+                        ok.store(false, std::memory_order_relaxed);
+                        break;
+                    }
+                }
+            }
+        });
+    for (auto& x : th) x.join();
+    return ok.load();
+}
+
+namespace {
+
+// The sequential sibling-group numbering below the top levels, for a tree:
+// the subtree of the i-th node of `roots` takes the contiguous range of ids
+// after those of roots 0..i-1 (8 ids per internal node in it), so the
+// subtrees are counted, then numbered, in parallel.  Returns the next id.
+uint32_t number_subtrees(const uint32_t* w, const std::vector<uint32_t>& roots, uint32_t next, uint32_t* new_id) {
+    const size_t m = roots.size();
+    std::vector<uint64_t> size(m, 0);
+    const unsigned T = worker_count();
+    auto run = [&](auto&& body) {
+        std::atomic<size_t> cursor{0};
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < T; ++t)
+            th.emplace_back([&] {
+                std::vector<uint32_t> stack;
+                for (size_t i; (i = cursor.fetch_add(1)) < m;) body(i, stack);
+            });
+        for (auto& x : th) x.join();
+    };
+    run([&](size_t i, std::vector<uint32_t>& stack) {
+        uint64_t count = 0;
+        stack.assign(1, roots[i]);
+        while (!stack.empty()) {
+            const uint32_t* nd = w + (uint64_t)stack.back() * kNodeWords;
+            stack.pop_back();
+            if (nd[9] & kLeafMask) continue;
+            count += 8;
+            for (int k = 0; k < 8; ++k) stack.push_back(nd[k]);
+        }
+        size[i] = count;
+    });
+    std::vector<uint64_t> start(m + 1, next);
+    for (size_t i = 0; i < m; ++i) start[i + 1] = start[i] + size[i];
+    run([&](size_t i, std::vector<uint32_t>& stack) {
+        uint64_t id = start[i];
+        stack.assign(1, roots[i]);
+        while (!stack.empty()) {
+            const uint32_t* nd = w + (uint64_t)stack.back() * kNodeWords;
+            stack.pop_back();
+            if (nd[9] & kLeafMask) continue;
+            for (int k = 0; k < 8; ++k) new_id[nd[k]] = (uint32_t)(id++);
+            for (int k = 7; k >= 0; --k) stack.push_back(nd[k]);  // child 0's group first
+        }
+    });
+    return (uint32_t)start[m];
+}
+
+}  // namespace
 
 void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels],
-                         bool wide) {
+                         bool wide, bool tree) {
     const uint32_t empty_tag = wide ? 0u : kTagEmpty;
     std::vector<uint32_t> new_id(n, 0xFFFFFFFFu);
     uint32_t next = 1;
@@ -51,24 +132,28 @@ void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint3
         top[L] = next;  // nodes of levels 0..L
     }
     // below: sibling groups, depth-first from each node of the last top level
-    std::vector<uint32_t> stack;
-    uint32_t fresh[8];
-    for (auto it = level.rbegin(); it != level.rend(); ++it) stack.push_back(*it);
-    while (!stack.empty()) {
-        const uint32_t old = stack.back();
-        stack.pop_back();
-        const uint32_t* nd = w + (uint64_t)old * kNodeWords;
-        if (nd[9] & kLeafMask) continue;
-        int nf = 0;
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t c = nd[k];
-            if (new_id[c] == 0xFFFFFFFFu) {
-                new_id[c] = next++;
-                fresh[nf++] = c;
+    if (tree) {
+        next = number_subtrees(w, level, next, new_id.data());
+    } else {
+        std::vector<uint32_t> stack;
+        uint32_t fresh[8];
+        for (auto it = level.rbegin(); it != level.rend(); ++it) stack.push_back(*it);
+        while (!stack.empty()) {
+            const uint32_t old = stack.back();
+            stack.pop_back();
+            const uint32_t* nd = w + (uint64_t)old * kNodeWords;
+            if (nd[9] & kLeafMask) continue;
+            int nf = 0;
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t c = nd[k];
+                if (new_id[c] == 0xFFFFFFFFu) {
+                    new_id[c] = next++;
+                    fresh[nf++] = c;
+                }
             }
+            // visit child 0's group first
+            for (int k = nf - 1; k >= 0; --k) stack.push_back(fresh[k]);
         }
-        // visit child 0's group first
-        for (int k = nf - 1; k >= 0; --k) stack.push_back(fresh[k]);
     }
     // nodes no internal node points to (never traversed) go last
     for (uint64_t i = 0; i < n; ++i)
@@ -132,7 +217,34 @@ void build_start_table(const uint32_t* child_w, uint32_t root_tag, uint32_t inde
     fill_table(child_w, index_mask, root_tag, 0, 0, 0, 0, table);
 }
 
-bool octree_depths_exact(const uint32_t* w, uint64_t n) {
+bool octree_depths_exact(const uint32_t* w, uint64_t n, bool tree) {
+    if (tree) {
+        // one parent per node: a reachable node's depth is its parent's + 1,
+        // so checking every internal node against its internal children (and
+        // the root against 0) is the walk's check, done in parallel; an
+        // unreachable internal node can only make the answer false
+        if (!(w[9] & kLeafMask) && (w[9] & kDepthMask) != 0) return false;
+        std::atomic<bool> ok{true};
+        const unsigned T = worker_count();
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                for (uint64_t i = t; i < n && ok.load(std::memory_order_relaxed); i += T) {
+                    const uint32_t* nd = w + i * kNodeWords;
+                    if (nd[9] & kLeafMask) continue;
+                    const uint32_t d = nd[9] & kDepthMask;
+                    for (int k = 0; k < 8; ++k) {
+                        const uint32_t* cn = w + (uint64_t)nd[k] * kNodeWords;
+                        if (!(cn[9] & kLeafMask) && (cn[9] & kDepthMask) != d + 1) {
+                            ok.store(false, std::memory_order_relaxed);
+                            break;
+                        }
+                    }
+                }
+            });
+        for (auto& x : th) x.join();
+        return ok.load();
+    }
     std::vector<uint8_t> depth(n, 0xFF);
     std::vector<uint32_t> level{0}, next;
     depth[0] = 0;

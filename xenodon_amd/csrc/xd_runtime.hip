@@ -10,7 +10,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "xenodon.h"
@@ -322,21 +325,50 @@ int xd_volume_from_grid(xd_context* ctx, const void* rgba8, uint64_t dim_x, uint
 
 // Structural depth of the tree below the root, and format checks.
 static int analyse_octree(const uint32_t* w, uint64_t n, uint32_t* max_depth, uint32_t* is_rope) {
-    bool ordered = true;
-    bool rope = false;
-    for (uint64_t i = 0; i < n; ++i) {
+    // one pass in slices over threads: the first bad node of the lowest slice
+    // that has one is reported, as a sequential scan would
+    const unsigned T = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(
+        std::min(16u, std::max(1u, std::thread::hardware_concurrency())), n >> 20));
+    std::vector<uint64_t> bad(T, UINT64_MAX);
+    std::vector<uint8_t> unordered(T, 0), ropes(T, 0);
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                const uint64_t lo = n * t / T, hi = n * (t + 1) / T;
+                for (uint64_t i = lo; i < hi; ++i) {
+                    const uint32_t* nd = w + i * kNodeWords;
+                    const bool leaf = (nd[9] & kLeafMask) != 0;
+                    bool ok = (nd[9] & kDepthMask) <= 64;
+                    for (int k = 0; k < 8; ++k) {
+                        ok = ok && nd[k] < n;
+                        if (!leaf && nd[k] <= i) unordered[t] = 1;
+                        if (leaf && k < 6 && nd[k] != 0) ropes[t] = 1;
+                    }
+                    if (!ok) {
+                        bad[t] = i;
+                        return;
+                    }
+                }
+            });
+        for (auto& x : th) x.join();
+    }
+    for (unsigned t = 0; t < T; ++t) {
+        if (bad[t] == UINT64_MAX) continue;
+        const uint64_t i = bad[t];
         const uint32_t* nd = w + i * kNodeWords;
-        const bool leaf = (nd[9] & kLeafMask) != 0;
         if ((nd[9] & kDepthMask) > 64)
             return fail(XD_ERR_FORMAT, "node %llu has depth field %u (above 64)", (unsigned long long)i,
                         nd[9] & kDepthMask);
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < 8; ++k)
             if (nd[k] >= n)
                 return fail(XD_ERR_FORMAT, "node %llu child %d = %u is out of range (%llu nodes)",
                             (unsigned long long)i, k, nd[k], (unsigned long long)n);
-            if (!leaf && nd[k] <= i) ordered = false;
-            if (leaf && k < 6 && nd[k] != 0) rope = true;
-        }
+    }
+    bool ordered = true, rope = false;
+    for (unsigned t = 0; t < T; ++t) {
+        ordered = ordered && !unordered[t];
+        rope = rope || ropes[t];
     }
     *is_rope = rope ? 1u : 0u;
     std::vector<uint8_t> height(n, 0);
@@ -405,32 +437,35 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
     v->bytes = bytes;
     if (node_count <= kMaxTaggedNodes && ctx->octree_layout != 1) {
         // tagged, sibling-grouped layout, derived on the host
-        std::vector<uint32_t> cw, meta;
-        try {
-            cw.resize(node_count * 8);
-            meta.resize(node_count * 2);
-        } catch (const std::bad_alloc&) {
+        // every word is written by the derivation: no zero-fill of tens of GB
+        std::unique_ptr<uint32_t[]> cw_buf(new (std::nothrow) uint32_t[node_count * 8]);
+        std::unique_ptr<uint32_t[]> meta_buf(new (std::nothrow) uint32_t[node_count * 2]);
+        if (!cw_buf || !meta_buf) {
             xd_volume_free(v);
             return fail(XD_ERR_OUT_OF_MEMORY, "out of host memory deriving the octree layout");
         }
+        uint32_t* const cw = cw_buf.get();
+        uint32_t* const meta = meta_buf.get();
         const bool wide = node_count > kMaxNarrowTagNodes || ctx->octree_layout == 2;
-        build_tagged_layout(w, node_count, cw.data(), meta.data(), v->top, wide);
-        v->depth_exact = octree_depths_exact(w, node_count) ? 1u : 0u;
-        v->rope_depth = !wide && pack_rope_depths(cw.data(), meta.data(), node_count) ? 1u : 0u;
+        // large trees (the converter's, not DAGs) are numbered and checked in parallel
+        const bool tree = node_count >= (1ull << 22) && octree_is_tree(w, node_count);
+        build_tagged_layout(w, node_count, cw, meta, v->top, wide, tree);
+        v->depth_exact = octree_depths_exact(w, node_count, tree) ? 1u : 0u;
+        v->rope_depth = !wide && pack_rope_depths(cw, meta, node_count) ? 1u : 0u;
         v->index_mask = v->rope_depth ? kRopeIndexMask : wide ? kWideIndexMask : kIndexMask;
         v->empty_tag = wide ? 0u : kTagEmpty;
         if ((e = hipMalloc(&v->d_cw, node_count * 8 * sizeof(uint32_t))) != hipSuccess ||
             (e = hipMalloc(&v->d_meta, node_count * sizeof(uint2))) != hipSuccess ||
-            (e = hipMemcpy(v->d_cw, cw.data(), node_count * 8 * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
+            (e = hipMemcpy(v->d_cw, cw, node_count * 8 * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
                 hipSuccess ||
-            (e = hipMemcpy(v->d_meta, meta.data(), node_count * sizeof(uint2), hipMemcpyHostToDevice)) != hipSuccess) {
+            (e = hipMemcpy(v->d_meta, meta, node_count * sizeof(uint2), hipMemcpyHostToDevice)) != hipSuccess) {
             xd_volume_free(v);
             return fail_hip(e, "octree upload");
         }
         const uint32_t root_ld = w[9], root_col = w[8];
         v->root_tag = (root_ld & kLeafMask) ? (kTagLeaf | ((root_col & 0xFFFFFFu) ? 0u : v->empty_tag)) : 0u;
         std::vector<uint2> table(kTableCells);
-        build_start_table(cw.data(), v->root_tag, v->index_mask, table.data());
+        build_start_table(cw, v->root_tag, v->index_mask, table.data());
         if ((e = hipMalloc(&v->d_table, kTableBytes)) != hipSuccess ||
             (e = hipMemcpy(v->d_table, table.data(), kTableBytes, hipMemcpyHostToDevice)) != hipSuccess) {
             xd_volume_free(v);

@@ -79,14 +79,18 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
 // Host-side derivation of the tagged, sibling-grouped device layout
 // (child_w: n*8 words, meta: n*2 words) from .svo node words.
 // top[L] = number of nodes in levels 0..L, which hold indices [0, top[L]).
-// wide: 31-bit indices and the leaf tag only (trees above 2^30 nodes).
+// wide: 31-bit indices and the leaf tag only (trees above 2^30 nodes).  tree:
+// octree_is_tree(w, n) holds, so the numbering runs subtree-parallel.
 void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels],
-                         bool wide);
+                         bool wide, bool tree);
+// True when no internal node's child word names the root and no node is
+// named by two child words of internal nodes (every node has one parent).
+bool octree_is_tree(const uint32_t* w, uint64_t n);
 // True when every internal node reachable from the root is reached only at
 // one depth and its is_leaf_depth depth field equals that depth (root = 0),
 // as in every tree the converter writes.  Traversals may then track depth
 // themselves instead of reading the field back.
-bool octree_depths_exact(const uint32_t* w, uint64_t n);
+bool octree_depths_exact(const uint32_t* w, uint64_t n, bool tree);
 // Start table of a tagged layout (child_w as build_tagged_layout writes it,
 // root_tag the root's own tags): entry (z << 12 | y << 6 | x) of the level-6
 // grid = {tagged word of the node find() reaches at level 6 for any point of
