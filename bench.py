@@ -45,6 +45,7 @@ from xenodon_amd import abi, synth, workloads  # noqa: E402
 
 CAMERA_FILE = os.path.join(ROOT, "tests", "golden", "camera.txt")
 WIDTH, HEIGHT = 1920, 1080
+SERVER_CONFIG = {}  # xd_server_config fields set from the command line (tiles_per_launch, launches_in_flight)
 EMISSION = 1.6  # <= 1 % saturated pixels on this volume and path
 TRAV_ORDER = ["dda", "svo-naive", "esvo", "svo-df", "svo-rope"]
 HBM_PEAK_GBS = 8000.0
@@ -187,7 +188,7 @@ def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plai
     load_served reuses resident volumes; otherwise a server is made for the
     leg."""
     if server is None:
-        srv = ctx.server(tuning=tuning)
+        srv = ctx.server(tuning=tuning, **SERVER_CONFIG)
         ids = load_served(srv, volumes)
     else:
         srv, ids = server
@@ -285,6 +286,10 @@ def main():
     ap.add_argument("--serve-rates", default="200,1000,2000,3000", help="Poisson request rates (req/s); empty: skip")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-tuning", default="", help="served-form tuning fields as JSON, over the defaults")
+    ap.add_argument("--serve-tiles-per-launch", type=int, default=0, help="served form: wave tiles per step "
+                                                                             "(0: the library's)")
+    ap.add_argument("--serve-in-flight", type=int, default=0, help="served form: steps queued on the device "
+                                                                   "(0: the library's)")
     ap.add_argument("--c5", action="store_true",
                     help="also serve requests over the 2048^3 and 1024^3 log-normal volumes next to the others")
     ap.add_argument("--c5-rates", default="50,200,500,1000,2000")
@@ -300,6 +305,7 @@ def main():
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         return 2
     travs = [t for t in args.traversals.split(",") if t]
+    SERVER_CONFIG.update(tiles_per_launch=args.serve_tiles_per_launch, launches_in_flight=args.serve_in_flight)
 
     t0 = time.time()
     grid = synth.ct_body()
@@ -564,7 +570,7 @@ def main():
                        "lognormal1024_svo": ("svo", (c3_host[1], 1024), [1024] * 3),
                        "lognormal2048_grid": ("grid", huge, [2048] * 3),
                        "lognormal2048_svo": ("svo", (hnodes, hst["side"]), [hst["side"]] * 3)})
-        srv = ctx.server(tuning=serve_tuning)
+        srv = ctx.server(tuning=serve_tuning, **SERVER_CONFIG)
         t0 = time.time()
         ids = load_served(srv, c5vols)
         log(f"served volumes resident ({srv.stats()['resident_bytes'] / 1e9:.1f} GB) in {time.time() - t0:.1f}s")

@@ -258,7 +258,7 @@ typedef struct {
     uint64_t frame_arena_bytes;  /* device memory for frames of admitted requests;
                                     0 = 512 MiB */
     uint32_t max_active;         /* requests admitted at once (1..1024; 0 = 256) */
-    uint32_t tiles_per_launch;   /* wave tiles per scheduling step (0 = 32768) */
+    uint32_t tiles_per_launch;   /* wave tiles per scheduling step (0 = 65536) */
     uint32_t launches_in_flight; /* 1..4 (0 = 2) */
     uint32_t readback_threads;   /* host threads copying finished frames out (1..16; 0 = 4) */
     xd_batch_tuning tuning;      /* tile shapes etc.; zero-filled = defaults */
