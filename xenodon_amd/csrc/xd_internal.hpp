@@ -77,6 +77,7 @@ struct RenderParams {
     uint32_t rope_depth;   // leaves' child words carry their targets' depth fields (kRopeDepthShift)
     uint32_t index_mask;   // index bits of a tagged child word (kIndexMask, kWideIndexMask, kRopeIndexMask)
     uint32_t empty_tag;    // kTagEmpty, or 0 for wide tags
+    uint32_t empty_start;  // wide tags: child indices from here on are empty leaves (else 0xFFFFFFFF)
     uint32_t xcd_chunks;  // plain launches: block runs per XCD (0: hardware order)
     unsigned long long* words;  // word counter (instrumented variant only)
 };

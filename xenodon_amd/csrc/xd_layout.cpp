@@ -10,10 +10,13 @@
 // In a DAG a shared node keeps the index of its first visit.  The first
 // kTopLevels levels are numbered breadth-first ahead of everything else, so
 // their child words form one small block that the kernels stage in LDS.
+// Leaves with zero RGB then move behind every other node, keeping order.
 //
 // child_w[i*8 + k] = new index of child k | kTagLeaf (child is a leaf)
 //                    | kTagEmpty (child is a leaf with zero RGB);
 // leaves' child words are their ropes, renumbered and tagged the same way.
+// Trees above 2^30 nodes use 31-bit indices and no empty tag: an index at or
+// past the first empty leaf's marks one.
 // meta[i] = {color, is_leaf_depth} of node i.
 #include <algorithm>
 #include <atomic>
@@ -104,10 +107,55 @@ uint32_t number_subtrees(const uint32_t* w, const std::vector<uint32_t>& roots, 
     return (uint32_t)start[m];
 }
 
+// Moves the empty leaves (zero RGB, other than the root) to the end of the
+// numbering, in order: a child word names an empty leaf exactly when its
+// index is >= *empty_start.  Every other node keeps its relative order, so
+// sibling groups stay together and top[] becomes the count of non-empty nodes
+// of the top levels.
+void partition_empty_leaves(const uint32_t* w, uint64_t n, uint32_t* new_id, uint32_t top[kTopLevels],
+                            uint32_t* empty_start) {
+    auto empty_leaf = [w](uint64_t i) {
+        const uint32_t* nd = w + i * kNodeWords;
+        return i != 0 && (nd[9] & kLeafMask) && !(nd[8] & 0xFFFFFFu);
+    };
+    std::vector<uint32_t> order(n);  // old index by current new index
+    const unsigned T = worker_count();
+    auto parallel = [T](uint64_t count, auto&& body) {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < T; ++t) th.emplace_back([&, t] { body(count * t / T, count * (t + 1) / T, t); });
+        for (auto& x : th) x.join();
+    };
+    parallel(n, [&](uint64_t lo, uint64_t hi, unsigned) {
+        for (uint64_t i = lo; i < hi; ++i) order[new_id[i]] = (uint32_t)i;
+    });
+    // non-empty nodes before each slice of the current numbering
+    std::vector<uint64_t> kept(T + 1, 0);
+    parallel(n, [&](uint64_t lo, uint64_t hi, unsigned t) {
+        uint64_t c = 0;
+        for (uint64_t k = lo; k < hi; ++k) c += empty_leaf(order[k]) ? 0 : 1;
+        kept[t + 1] = c;
+    });
+    for (unsigned t = 0; t < T; ++t) kept[t + 1] += kept[t];
+    const uint64_t non_empty = kept[T];
+    parallel(n, [&](uint64_t lo, uint64_t hi, unsigned t) {
+        uint64_t a = kept[t], e = non_empty + (lo - kept[t]);
+        for (uint64_t k = lo; k < hi; ++k) {
+            const uint32_t i = order[k];
+            new_id[i] = (uint32_t)(empty_leaf(i) ? e++ : a++);
+        }
+    });
+    for (uint32_t L = 0; L < kTopLevels; ++L) {
+        uint32_t c = 0;
+        for (uint32_t k = 0; k < top[L]; ++k) c += empty_leaf(order[k]) ? 0 : 1;
+        top[L] = c;
+    }
+    *empty_start = (uint32_t)non_empty;
+}
+
 }  // namespace
 
 void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels],
-                         bool wide, bool tree) {
+                         bool wide, bool tree, uint32_t* empty_start) {
     const uint32_t empty_tag = wide ? 0u : kTagEmpty;
     std::vector<uint32_t> new_id(n, 0xFFFFFFFFu);
     uint32_t next = 1;
@@ -158,6 +206,11 @@ void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint3
     // nodes no internal node points to (never traversed) go last
     for (uint64_t i = 0; i < n; ++i)
         if (new_id[i] == 0xFFFFFFFFu) new_id[i] = next++;
+    // empty leaves last: with wide tags their indices mark them; with narrow
+    // tags the tag does, and the move still packs the other nodes closer
+    // (svo-naive and esvo about 2 % faster on the 512x361x512 tree)
+    partition_empty_leaves(w, n, new_id.data(), top, empty_start);
+    if (!wide) *empty_start = 0xFFFFFFFFu;
 
     const unsigned T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> th;

@@ -110,6 +110,7 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
         p.rope_depth = vol->rope_depth;
         p.index_mask = vol->index_mask;
         p.empty_tag = vol->empty_tag;
+        p.empty_start = vol->empty_start;
         p.root_tag = vol->root_tag;
         // levels 0..3 in LDS for svo-rope only: svo-naive starts every walk from
         // the start table, and esvo / svo-df run at full occupancy with their
@@ -450,7 +451,7 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
         const bool wide = node_count > kMaxNarrowTagNodes || ctx->octree_layout == 2;
         // large trees (the converter's, not DAGs) are numbered and checked in parallel
         const bool tree = node_count >= (1ull << 22) && octree_is_tree(w, node_count);
-        build_tagged_layout(w, node_count, cw, meta, v->top, wide, tree);
+        build_tagged_layout(w, node_count, cw, meta, v->top, wide, tree, &v->empty_start);
         v->depth_exact = octree_depths_exact(w, node_count, tree) ? 1u : 0u;
         v->rope_depth = !wide && pack_rope_depths(cw, meta, node_count) ? 1u : 0u;
         v->index_mask = v->rope_depth ? kRopeIndexMask : wide ? kWideIndexMask : kIndexMask;

@@ -50,6 +50,7 @@ struct xd_volume {
     uint32_t rope_depth = 0;   // ropes carry their targets' depth fields (see pack_rope_depths)
     uint32_t index_mask = 0;   // index bits of the tagged child words
     uint32_t empty_tag = 0;    // kTagEmpty, or 0 for wide tags
+    uint32_t empty_start = 0xFFFFFFFFu;  // wide tags: empty leaves are the indices from here on
     uint32_t root_tag = 0;
     uint32_t depth_exact = 0;  // internal nodes' depth fields match their depth (see octree_depths_exact)
     uint32_t top[xd::kTopLevels] = {0, 0, 0, 0};  // nodes in breadth-first levels 0..L
@@ -79,10 +80,13 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
 // Host-side derivation of the tagged, sibling-grouped device layout
 // (child_w: n*8 words, meta: n*2 words) from .svo node words.
 // top[L] = number of nodes in levels 0..L, which hold indices [0, top[L]).
-// wide: 31-bit indices and the leaf tag only (trees above 2^30 nodes).  tree:
-// octree_is_tree(w, n) holds, so the numbering runs subtree-parallel.
+// wide: 31-bit indices and the leaf tag only (trees above 2^30 nodes).  The
+// empty leaves take the last indices; with wide tags *empty_start is the
+// first of them, which is how a child word marks an empty leaf (else
+// *empty_start = 0xFFFFFFFF and the empty tag does).  tree: octree_is_tree(w, n) holds, so the
+// numbering runs subtree-parallel.
 void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels],
-                         bool wide, bool tree);
+                         bool wide, bool tree, uint32_t* empty_start);
 // True when no internal node's child word names the root and no node is
 // named by two child words of internal nodes (every node has one parent).
 bool octree_is_tree(const uint32_t* w, uint64_t n);

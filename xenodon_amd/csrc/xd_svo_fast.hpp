@@ -31,7 +31,13 @@ struct FastSvo {
     uint32_t rope_depth;   // ropes carry their targets' depth fields (kRopeDepthShift)
     uint32_t index_mask;   // index bits of a child word
     uint32_t empty_tag;    // kTagEmpty, or 0 where leaves carry no empty tag
+    uint32_t empty_start;  // ... where they do not, the empty leaves' first index
 };
+
+// A child word names a leaf with zero RGB (whose colour need not be loaded).
+__device__ __forceinline__ bool empty_leaf(const FastSvo& s, uint32_t wd) {
+    return (wd & s.empty_tag) || (wd & s.index_mask) >= s.empty_start;
+}
 
 // Child word k of node n: from LDS for the breadth-first top levels.
 __device__ __forceinline__ uint32_t cw_at(const FastSvo& s, uint32_t n, uint32_t k) {
@@ -44,7 +50,7 @@ __device__ __forceinline__ uint32_t cw_at(const FastSvo& s, uint32_t n, uint32_t
 __device__ __forceinline__ float pmod(float x, float y, float inv_y) { return x - y * __builtin_floorf(x * inv_y); }
 
 __device__ __forceinline__ uint32_t colour_if(const FastSvo& s, uint32_t node, uint32_t tag) {
-    return (tag & s.empty_tag) ? 0u : s.meta[node].x;
+    return empty_leaf(s, tag) ? 0u : s.meta[node].x;
 }
 
 // The child choices of find() at every level, packed: bit (22 - l) of
@@ -261,7 +267,7 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
             const uint32_t wd = pick8(lo, hi, ci);
             const bool bx = (ci & 4u) != 0, by = (ci & 2u) != 0, bz = (ci & 1u) != 0;
             if (wd & kTagLeaf) {
-                if (!(wd & s.empty_tag)) {  // an empty leaf adds exactly +0
+                if (!empty_leaf(s, wd)) {  // an empty leaf adds exactly +0
                     float t_min, t_max;
                     child_trange(corner, side, rrd, bias, ci, t_min, t_max);
                     total = add(total, muls(unorm_rgb(s.meta[wd & s.index_mask].x), t_max - gmax(t_min, 0.0f)));
@@ -387,7 +393,7 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
         u_max = min_elem(far);
         step = u_max - gmax(u_min, 0.0f);
         const uint32_t col =
-            (tag & s.empty_tag) ? 0u : (node == start && !s.rope_depth ? start_col : s.meta[node].x);
+            empty_leaf(s, tag) ? 0u : (node == start && !s.rope_depth ? start_col : s.meta[node].x);
         n = neighbor_index(nbx, nby, nbz, far, mask);
         wd = cw_at(s, node, n);  // in flight together with the colour
         total = add(total, muls(unorm_rgb(col), step));
