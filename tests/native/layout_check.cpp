@@ -2,10 +2,10 @@
 // against the .svo words it is derived from.  Usage:
 //   layout_check <nodes.words> <wide 0|1> <points>
 // Reads node_count * 10 little-endian u32 words and checks:
-//   * the numbering: node 0 stays 0, the top levels come first, empty leaves
-//     last, every child word names the renumbered child (the walk a plain
-//     sequential numbering would give, recomputed here), tags = the child's
-//     leaf / empty-leaf state;
+//   * the numbering: node 0 stays 0, internal nodes first (the top levels
+//     ahead), then leaves with colour, then empty leaves; every child word
+//     names the renumbered child (the walk a plain sequential numbering would
+//     give, recomputed here), tags = the child's leaf / empty-leaf state;
 //   * meta[new(i)] = {color, is_leaf_depth} of node i;
 //   * octree_depths_exact: the subtree-parallel check equals the breadth-first
 //     walk on trees;
@@ -153,16 +153,22 @@ int main(int argc, char** argv) {
     build_tagged_layout(w.data(), n, cw.data(), meta.data(), top, wide, tree, &empty_start);
     std::vector<uint32_t> id = reference_ids(w.data(), n);
     {
-        // empty leaves (not the root) move behind every other node, in order
+        // internal nodes (and the root), then leaves with colour, then empty
+        // leaves, each in the order of the sequential numbering
         std::vector<uint32_t> order(n);
         for (uint64_t i = 0; i < n; ++i) order[id[i]] = (uint32_t)i;
-        auto empty = [&](uint32_t i) { return i != 0 && (w[(uint64_t)i * 10 + 9] & kLeafMask) && !(w[(uint64_t)i * 10 + 8] & 0xFFFFFFu); };
-        uint32_t kept = 0;
-        for (uint64_t k = 0; k < n; ++k) kept += empty(order[k]) ? 0 : 1;
+        auto cls = [&](uint32_t i) {
+            const uint32_t* nd = w.data() + (uint64_t)i * kNodeWords;
+            if (i == 0 || !(nd[9] & kLeafMask)) return 0;
+            return (nd[8] & 0xFFFFFFu) ? 1 : 2;
+        };
+        uint32_t count[3] = {0, 0, 0};
+        for (uint64_t k = 0; k < n; ++k) ++count[cls(order[k])];
+        const uint32_t kept = count[0] + count[1];
         CHECK(empty_start == (wide ? kept : 0xFFFFFFFFu), "empty_start %u (wide %d, %u non-empty)", empty_start,
               (int)wide, kept);
-        uint32_t a = 0, e = kept;
-        for (uint64_t k = 0; k < n; ++k) id[order[k]] = empty(order[k]) ? e++ : a++;
+        uint32_t at[3] = {0, count[0], kept};
+        for (uint64_t k = 0; k < n; ++k) id[order[k]] = at[cls(order[k])]++;
     }
     const uint32_t empty_tag = wide ? 0u : kTagEmpty;
     for (uint64_t i = 0; i < n; ++i) {

@@ -10,7 +10,8 @@
 // In a DAG a shared node keeps the index of its first visit.  The first
 // kTopLevels levels are numbered breadth-first ahead of everything else, so
 // their child words form one small block that the kernels stage in LDS.
-// Leaves with zero RGB then move behind every other node, keeping order.
+// Finally the numbering is split, keeping order within each part: internal
+// nodes, then leaves with colour, then leaves with zero RGB.
 //
 // child_w[i*8 + k] = new index of child k | kTagLeaf (child is a leaf)
 //                    | kTagEmpty (child is a leaf with zero RGB);
@@ -107,16 +108,19 @@ uint32_t number_subtrees(const uint32_t* w, const std::vector<uint32_t>& roots, 
     return (uint32_t)start[m];
 }
 
-// Moves the empty leaves (zero RGB, other than the root) to the end of the
-// numbering, in order: a child word names an empty leaf exactly when its
-// index is >= *empty_start.  Every other node keeps its relative order, so
-// sibling groups stay together and top[] becomes the count of non-empty nodes
-// of the top levels.
-void partition_empty_leaves(const uint32_t* w, uint64_t n, uint32_t* new_id, uint32_t top[kTopLevels],
+// Stable three-way partition of the numbering: internal nodes (and the root)
+// first, then leaves with colour, then leaves with zero RGB.  The child words
+// that descents read (those of internal nodes) become one dense block, and a
+// child word names an empty leaf exactly when its index is >= *empty_start.
+// top[] becomes the count of internal nodes of the top levels (whose child
+// words the rope kernel stages in LDS).
+void partition_leaves(const uint32_t* w, uint64_t n, uint32_t* new_id, uint32_t top[kTopLevels],
                             uint32_t* empty_start) {
-    auto empty_leaf = [w](uint64_t i) {
+    // class 0: internal nodes (and the root), 1: leaves with colour, 2: empty leaves
+    auto cls = [w](uint64_t i) -> int {
         const uint32_t* nd = w + i * kNodeWords;
-        return i != 0 && (nd[9] & kLeafMask) && !(nd[8] & 0xFFFFFFu);
+        if (i == 0 || !(nd[9] & kLeafMask)) return 0;
+        return (nd[8] & 0xFFFFFFu) ? 1 : 2;
     };
     std::vector<uint32_t> order(n);  // old index by current new index
     const unsigned T = worker_count();
@@ -128,28 +132,29 @@ void partition_empty_leaves(const uint32_t* w, uint64_t n, uint32_t* new_id, uin
     parallel(n, [&](uint64_t lo, uint64_t hi, unsigned) {
         for (uint64_t i = lo; i < hi; ++i) order[new_id[i]] = (uint32_t)i;
     });
-    // non-empty nodes before each slice of the current numbering
-    std::vector<uint64_t> kept(T + 1, 0);
+    std::vector<uint64_t> cnt(3 * (T + 1), 0);  // per class, nodes before each slice
     parallel(n, [&](uint64_t lo, uint64_t hi, unsigned t) {
-        uint64_t c = 0;
-        for (uint64_t k = lo; k < hi; ++k) c += empty_leaf(order[k]) ? 0 : 1;
-        kept[t + 1] = c;
+        uint64_t c[3] = {0, 0, 0};
+        for (uint64_t k = lo; k < hi; ++k) ++c[cls(order[k])];
+        for (int q = 0; q < 3; ++q) cnt[q * (T + 1) + t + 1] = c[q];
     });
-    for (unsigned t = 0; t < T; ++t) kept[t + 1] += kept[t];
-    const uint64_t non_empty = kept[T];
+    for (int q = 0; q < 3; ++q)
+        for (unsigned t = 0; t < T; ++t) cnt[q * (T + 1) + t + 1] += cnt[q * (T + 1) + t];
+    const uint64_t base[3] = {0, cnt[T], cnt[T] + cnt[(T + 1) + T]};
     parallel(n, [&](uint64_t lo, uint64_t hi, unsigned t) {
-        uint64_t a = kept[t], e = non_empty + (lo - kept[t]);
+        uint64_t at[3];
+        for (int q = 0; q < 3; ++q) at[q] = base[q] + cnt[q * (T + 1) + t];
         for (uint64_t k = lo; k < hi; ++k) {
             const uint32_t i = order[k];
-            new_id[i] = (uint32_t)(empty_leaf(i) ? e++ : a++);
+            new_id[i] = (uint32_t)(at[cls(i)]++);
         }
     });
     for (uint32_t L = 0; L < kTopLevels; ++L) {
         uint32_t c = 0;
-        for (uint32_t k = 0; k < top[L]; ++k) c += empty_leaf(order[k]) ? 0 : 1;
+        for (uint32_t k = 0; k < top[L]; ++k) c += cls(order[k]) == 0 ? 1 : 0;
         top[L] = c;
     }
-    *empty_start = (uint32_t)non_empty;
+    *empty_start = (uint32_t)base[2];
 }
 
 }  // namespace
@@ -206,10 +211,11 @@ void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint3
     // nodes no internal node points to (never traversed) go last
     for (uint64_t i = 0; i < n; ++i)
         if (new_id[i] == 0xFFFFFFFFu) new_id[i] = next++;
-    // empty leaves last: with wide tags their indices mark them; with narrow
-    // tags the tag does, and the move still packs the other nodes closer
-    // (svo-naive and esvo about 2 % faster on the 512x361x512 tree)
-    partition_empty_leaves(w, n, new_id.data(), top, empty_start);
+    // internal nodes, then leaves with colour, then empty leaves: with wide
+    // tags the empty leaves' indices mark them; with narrow tags the tag does,
+    // and the order still packs the descents' child words together
+    // (svo-naive +4 %, esvo +2 % on the 512x361x512 tree)
+    partition_leaves(w, n, new_id.data(), top, empty_start);
     if (!wide) *empty_start = 0xFFFFFFFFu;
 
     const unsigned T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
