@@ -23,6 +23,13 @@ octree).  Inputs are HBM-resident before timing.
            requests released at once, also the plain form on the same
            requests.
 
+--workload picks the configuration: ct512 (the default, above), shells128
+(the 128^3 shells grid, dda at 256x256 over a 10-frame orbit at twice the
+unit-cube diagonal, cycled; 150 frames per batched launch) or lognormal1024
+(the 1024^3 log-normal grid and its rope SVO built on the GPU, dda and esvo at
+3840x2160 over camera.txt).  The mixed-view and served legs run with ct512
+only.
+
 The roofline object covers the dominant kernel: algorithmic bytes (4 B per
 volume word the reference algorithm reads, counted by the instrumented kernel
 variant on the same frames, + 4 B per pixel) over its mean launch time.
@@ -49,7 +56,37 @@ SERVER_CONFIG = {}  # xd_server_config fields set from the command line (tiles_p
 EMISSION = 1.6  # <= 1 % saturated pixels on this volume and path
 TRAV_ORDER = ["dda", "svo-naive", "esvo", "svo-df", "svo-rope"]
 HBM_PEAK_GBS = 8000.0
-PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_ct512_1080p.json")
+# --workload: image size, traversals, emission coefficient, camera set and
+# the committed rocprofv3 traffic summary of each measured configuration
+WORKLOADS = {
+    "ct512": {"size": (1920, 1080), "traversals": TRAV_ORDER, "emission": EMISSION, "cameras": "path",
+              "steps": 150, "batch_frames": 10, "profile": "hbm_traffic_ct512_1080p.json",
+              "desc": "512x361x512 RGBA8 grid, all five traversals at 1920x1080, camera.txt path",
+              "data": "synthetic (seeded CT-like 512x361x512 RGBA8 grid + its lossless rope SVO; reference "
+                      "camera.txt)"},
+    "shells128": {"size": (256, 256), "traversals": ["dda"], "emission": 4.0, "cameras": "orbit",
+                  "steps": 1500, "batch_frames": 150, "profile": "hbm_traffic_shells128_256.json",
+                  "desc": "128^3 RGBA8 shells grid, dda at 256x256, 10-frame circular orbit at twice the "
+                          "unit-cube diagonal (cycled)",
+                  "data": "synthetic (seeded 128^3 RGBA8 grid: 24 ellipsoidal shells + 5 % noise voxels)"},
+    "lognormal1024": {"size": (3840, 2160), "traversals": ["dda", "esvo"], "emission": 1.0, "cameras": "path",
+                      "steps": 150, "batch_frames": 10, "profile": "hbm_traffic_lognormal1024_2160p.json",
+                      "desc": "1024^3 RGBA8 log-normal grid, dda on the grid and esvo on its rope SVO (one "
+                              "allocation above 4 GB) at 3840x2160, camera.txt path",
+                      "data": "synthetic (seeded 1024^3 log-normal RGBA8 grid, P(k) ~ k^-3, + its lossless rope "
+                              "SVO built on the GPU; reference camera.txt)"},
+}
+
+
+def orbit_cameras(n=10):
+    """n frames on a circular orbit around the unit cube's centre at twice
+    its diagonal, looking at the centre."""
+    cams = []
+    for k in range(n):
+        th = 2 * np.pi * k / n
+        eye = np.array([0.5, 0.5, 0.5]) + 2 * np.sqrt(3) * np.array([np.sin(th), 0.0, -np.cos(th)])
+        cams.append(workloads.look_at(eye, (0.5, 0.5, 0.5)))
+    return cams
 
 
 def log(msg):
@@ -67,32 +104,33 @@ def read_cameras(path):
     return [(r[0:3], r[3:6], r[6:9]) for r in v]
 
 
-def cpu_baseline(grid, nodes, side, cams, frames, threads, budget_s=30.0):
-    """The oracle on the host cores: full 1920x1080 frames of the same path
-    (frames evenly spaced over it), all five traversals per frame, stopping
-    after the frame that crosses budget_s seconds."""
+def cpu_baseline(grid, nodes, side, cams, frames, threads, budget_s=30.0, width=WIDTH, height=HEIGHT,
+                 emission=EMISSION, travs=TRAV_ORDER):
+    """The oracle on the host cores: full frames of the same path (frames
+    evenly spaced over it), every traversal of the workload per frame,
+    stopping after the frame that crosses budget_s seconds."""
     from oracle import xo  # the checker, timed here only as the CPU baseline
 
     og = xo.OracleVolume(grid=grid)
-    osv = xo.OracleVolume(nodes=nodes, side=side)
+    osv = xo.OracleVolume(nodes=nodes, side=side) if any(n != "dda" for n in travs) else None
     rays = 0
     secs = 0.0
-    per_rays = {n: 0 for n in TRAV_ORDER}
-    per_secs = {n: 0.0 for n in TRAV_ORDER}
+    per_rays = {n: 0 for n in travs}
+    per_secs = {n: 0.0 for n in travs}
     picked = [int(round(i * len(cams) / frames)) % len(cams) for i in range(frames)]
     done = []
     for k in picked:
         f, up, eye = cams[k]
-        for name in TRAV_ORDER:
+        for name in travs:
             t = abi.TRAVERSALS[name]
             vol, md = (og, list(grid.shape[2::-1][:3])) if t == abi.DDA else (osv, [side] * 3)
-            u = xo.make_uniforms(WIDTH, HEIGHT, md, emission=EMISSION)
+            u = xo.make_uniforms(width, height, md, emission=emission)
             t0 = time.perf_counter()
             xo.render(t, vol, xo.make_camera(f, up, eye), u, threads=threads)
             dt = time.perf_counter() - t0
-            rays += WIDTH * HEIGHT
+            rays += width * height
             secs += dt
-            per_rays[name] += WIDTH * HEIGHT
+            per_rays[name] += width * height
             per_secs[name] += dt
         done.append(k)
         if secs > budget_s:
@@ -103,8 +141,9 @@ def cpu_baseline(grid, nodes, side, cams, frames, threads, budget_s=30.0):
         "cores": threads,
         "kind": "port",
         "sample": f"oracle FP32 restatement on {threads} threads: camera frames {done} of the path at "
-                  f"{WIDTH}x{HEIGHT}, all 5 traversals each ({rays} rays, {secs:.1f} s)",
-        "per_traversal_mrays": {n: round(per_rays[n] / per_secs[n] / 1e6, 3) for n in TRAV_ORDER if per_secs[n]},
+                  f"{width}x{height}, {len(travs)} traversal{'s' if len(travs) > 1 else ''} each ({rays} rays, "
+                  f"{secs:.1f} s)",
+        "per_traversal_mrays": {n: round(per_rays[n] / per_secs[n] / 1e6, 3) for n in travs if per_secs[n]},
     }
 
 
@@ -256,10 +295,13 @@ def copy_calibration(gib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=150, help="camera frames timed (each frame: 5 traversals)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="camera frames timed, each with every traversal of the workload (default: 150; 1500 for "
+                         "shells128, its 10-frame orbit cycled)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--frame-stride", type=int, default=1, help="step k renders camera frame k*stride")
-    ap.add_argument("--batch-frames", type=int, default=10, help="camera frames per batched launch")
+    ap.add_argument("--batch-frames", type=int, default=None,
+                    help="camera frames per batched launch (default: 10; 150 for shells128's 256x256 frames)")
     ap.add_argument("--per-traversal-groups", type=int, default=3,
                     help="launches timed per traversal for the per-traversal batched rates (0: skip)")
     ap.add_argument("--autotune", action="store_true",
@@ -268,7 +310,11 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="the default; kept for older command lines")
     ap.add_argument("--tuning", default="", help="batch tuning fields as JSON, applied over the defaults "
                                                  "(without --autotune)")
-    ap.add_argument("--traversals", default=",".join(TRAV_ORDER))
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="ct512",
+                    help="ct512 (default): 512x361x512 volume, five traversals at 1920x1080; shells128: 128^3, "
+                         "dda at 256x256 on a 10-frame orbit; lognormal1024: 1024^3, dda and esvo at 3840x2160. "
+                         "The mixed-view and served legs run with ct512 only")
+    ap.add_argument("--traversals", default="", help="comma-separated subset (default: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8, help="camera frames in the CPU-baseline sample")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
@@ -304,33 +350,53 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         return 2
-    travs = [t for t in args.traversals.split(",") if t]
+    wl = WORKLOADS[args.workload]
+    width, height = wl["size"]
+    if args.steps is None:
+        args.steps = wl["steps"]
+    if args.batch_frames is None:
+        args.batch_frames = wl["batch_frames"]
+    emission = wl["emission"]
+    travs = [t for t in args.traversals.split(",") if t] or list(wl["traversals"])
     SERVER_CONFIG.update(tiles_per_launch=args.serve_tiles_per_launch, launches_in_flight=args.serve_in_flight)
-
-    t0 = time.time()
-    grid = synth.ct_body()
-    log(f"volume 512x361x512 generated in {time.time() - t0:.1f}s, fnv64 {synth.volume_hash(grid):016x}")
-    t0 = time.time()
-    nodes, st = abi.build_octree(grid, tree=abi.TREE_ROPE)
-    side = st["side"]
-    log(f"lossless rope SVO: {st['node_count']} nodes, depth {st['depth']}, built in {time.time() - t0:.1f}s")
-    cams = read_cameras(CAMERA_FILE)
+    if args.workload != "ct512":
+        # the mixed-view and served legs are workloads of their own, run
+        # with the default volume
+        args.mixed_views_reps = 0
+        args.serve_rates = ""
+        args.serve_saturated = 0
+        args.c5 = False
 
     ctx = abi.Context(0)
     if args.tile:
         ctx.set_tile(args.tile)
     if args.xcd_chunks >= 0:
         ctx.set_option("xcd_chunks", args.xcd_chunks)
+    t0 = time.time()
+    grid = {"ct512": synth.ct_body, "shells128": synth.shells_128,
+            "lognormal1024": lambda: synth.lognormal(side=1024)}[args.workload]()
+    dims = list(grid.shape[2::-1][:3])
+    log(f"volume {'x'.join(map(str, dims))} generated in {time.time() - t0:.1f}s, fnv64 "
+        f"{synth.volume_hash(grid):016x}")
+    t0 = time.time()
     gvol = ctx.grid_volume(grid)
+    if args.workload == "lognormal1024":
+        nodes, st = ctx.build_octree(gvol, tree=abi.TREE_ROPE)  # the GPU builder (bit-identical nodes)
+    else:
+        nodes, st = abi.build_octree(grid, tree=abi.TREE_ROPE)
+    side = st["side"]
+    log(f"lossless rope SVO: {st['node_count']} nodes, depth {st['depth']}, built in {time.time() - t0:.1f}s")
+    cams = read_cameras(CAMERA_FILE) if wl["cameras"] == "path" else orbit_cameras()
+
     svol = ctx.svo_volume(nodes, side)
-    frame = ctx.frame(WIDTH, HEIGHT)
-    vols = {"dda": (gvol, [512, 361, 512])}
+    frame = ctx.frame(width, height)
+    vols = {"dda": (gvol, dims)}
     for n in TRAV_ORDER[1:]:
         vols[n] = (svol, [side] * 3)
     log(f"resident: grid {gvol.info().device_bytes / 1e6:.0f} MB, svo {svol.info().device_bytes / 1e6:.0f} MB")
 
     def uni(md):
-        return abi.make_uniforms(WIDTH, HEIGHT, md, emission=EMISSION)
+        return abi.make_uniforms(width, height, md, emission=emission)
 
     def frame_of(k):
         return cams[(k * args.frame_stride) % len(cams)]
@@ -369,7 +435,7 @@ def main():
             step(k, ms_acc)
         plain_wall = time.perf_counter() - t_start
     log(f"plain: {args.steps} steps in {plain_wall:.2f}s")
-    rays_per_frame = WIDTH * HEIGHT
+    rays_per_frame = width * height
     total_rays = rays_per_frame * args.steps * len(travs)
     plain_value = total_rays / plain_wall / 1e6
     per_trav = {n: rays_per_frame * len(v) / (sum(v) * 1e3) for n, v in ms_acc.items()}
@@ -402,7 +468,7 @@ def main():
     # ---------------- batched form: one persistent launch per group --------
     # a group is `batch_frames` consecutive steps x the traversals = views
     B = max(1, min(args.batch_frames, args.steps))
-    frames_pool = [ctx.frame(WIDTH, HEIGHT) for _ in range(B * len(travs))]
+    frames_pool = [ctx.frame(width, height) for _ in range(B * len(travs))]
     batch = ctx.batch()
 
     def group_views(k0):
@@ -478,11 +544,12 @@ def main():
 
     traffic = None
     measured = {}
-    if os.path.exists(PROFILE_SUMMARY):
+    profile_summary = os.path.join(ROOT, "profiles", wl["profile"])
+    if os.path.exists(profile_summary):
         # HBM bytes per launch of the same launch shape from the committed
         # rocprofv3 counter runs; over this run's launch time they give the
         # physically bounded rate next to the algorithmic one
-        prof = json.load(open(PROFILE_SUMMARY))
+        prof = json.load(open(profile_summary))
         traffic = prof.get("hbm_bytes_per_launch", {}).get("batch")
         k = prof.get("per_kernel", {}).get("batch", {})
         if traffic:
@@ -507,7 +574,7 @@ def main():
     if args.mixed_views_reps > 0:
         sg, ss = ctx.grid_volume(shells), ctx.svo_volume(snodes, sst["side"])
         scenes = {"shells128": (sg, [128] * 3, ss, [sst["side"]] * 3, 4.0),
-                  "ct512": (gvol, [512, 361, 512], svol, [side] * 3, EMISSION)}
+                  "ct512": (gvol, [512, 361, 512], svol, [side] * 3, emission)}
         extra = []
         if args.mixed_views_large:
             t0 = time.time()
@@ -590,7 +657,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", 0) or min(16, os.cpu_count() or 1))
-        cpu = cpu_baseline(grid, nodes, side, cams, args.cpu_frames, threads)
+        cpu = cpu_baseline(grid, nodes, side, cams, args.cpu_frames, threads, width=width, height=height,
+                           emission=emission, travs=travs)
 
     out = {
         "metric": "Mrays/s per traversal @1080p, batched/served vs plain; p50/p99 request latency",
@@ -604,10 +672,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded CT-like 512x361x512 RGBA8 grid + its lossless rope SVO; reference camera.txt)",
-        "config": {"workload": "512x361x512 RGBA8 grid, all five traversals at 1920x1080, camera.txt path",
+        "data": wl["data"],
+        "config": {"workload": wl["desc"],
                    "form": f"batched: persistent ray-tile queue, {B} frames x {len(travs)} traversals per "
-                           "hipGraph launch", "emission_coeff": EMISSION,
+                           "hipGraph launch", "emission_coeff": emission,
                    "saturated_pixel_fraction": saturated, "traversals": travs,
                    "frames": len(groups) * B, "batch_tuning": tuning},
         "batched_vs_plain": round(value / plain_value, 3),
