@@ -71,6 +71,10 @@ WORKLOADS = {
                   "data": "synthetic (seeded 128^3 RGBA8 grid: 24 ellipsoidal shells + 5 % noise voxels)"},
     "lognormal1024": {"size": (3840, 2160), "traversals": ["dda", "esvo"], "emission": 1.0, "cameras": "path",
                       "steps": 150, "batch_frames": 10, "profile": "hbm_traffic_lognormal1024_2160p.json",
+                      # the autotuner's pick here: one persistent kernel per traversal, run in
+                      # turn, so the grid's and the octree's working sets do not share the
+                      # caches
+                      "tuning": {"class_kernels": 1},
                       "desc": "1024^3 RGBA8 log-normal grid, dda on the grid and esvo on its rope SVO (one "
                               "allocation above 4 GB) at 3840x2160, camera.txt path",
                       "data": "synthetic (seeded 1024^3 log-normal RGBA8 grid, P(k) ~ k^-3, + its lossless rope "
@@ -490,8 +494,10 @@ def main():
             vol, md = vols[name]
             tune_views.append((vol, abi.TRAVERSALS[name], cam, uni(md), frames_pool[j * len(travs) + i]))
     batch.set_views(tune_views)
-    if args.tuning:
-        t = dict(batch.tuning(), **json.loads(args.tuning))
+    if args.tuning or (wl.get("tuning") and not args.autotune):
+        t = dict(batch.tuning())
+        t.update(wl.get("tuning", {}))
+        t.update(json.loads(args.tuning or "{}"))
         batch.set_tuning(t["tile_w"], t["tiles_per_grab"], t["waves_per_simd"], t["class_kernels"], t["interleave"],
                          t["mix_classes"])
     tuning = batch.autotune(trials=2) if args.autotune else batch.tuning()

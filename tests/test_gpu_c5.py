@@ -95,3 +95,31 @@ def test_c5_served_requests_match_plain_and_oracle(c5):
     assert oracle_checked == {abi.DDA, abi.ESVO}
     plain_vols["shells_grid"].free()
     plain_vols["shells_svo"].free()
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("waves", [8, 0])
+def test_c5_dda_class_kernel_batch_matches_plain(c5, waves):
+    """The DDA class kernel (macro words of the next steps loaded during the
+    current gathers) on the 2048^3 grid, whose texel offsets need 64 bits:
+    every batched frame equals the plain render."""
+    ctx, grid, gv, nodes, st, sv = c5
+    rng = np.random.default_rng(77)
+    views = []
+    for W, H in ((640, 360), (256, 256), (333, 217)):
+        cam = abi.make_camera(*workloads.orbit_camera(rng))
+        views.append((gv, abi.DDA, cam, abi.make_uniforms(W, H, [2048] * 3, emission=EMISSION), ctx.frame(W, H)))
+    b = ctx.batch()
+    tw = b.tuning()
+    b.set_tuning(tw["tile_w"], tw["tiles_per_grab"], waves, 1, tw["interleave"], tw["mix_classes"])
+    b.set_views(views)
+    b.render()
+    lit = 0
+    for vol, t, cam, u, f in views:
+        got = f.download()
+        ref, _ = ctx.render_image(vol, t, cam, u)
+        assert np.array_equal(got, ref)
+        lit += int(got[..., :3].any())
+        f.free()
+    assert lit >= 2
+    b.free()

@@ -88,7 +88,8 @@ def test_c3_raw_octree_words_with_64bit_offsets(c3):
 
 
 @pytest.mark.timeout(600)
-@pytest.mark.parametrize("tuning", [None, {"interleave": 2, "waves_per_simd": 8, "mix_classes": 1}])
+@pytest.mark.parametrize("tuning", [None, {"interleave": 2, "waves_per_simd": 8, "mix_classes": 1},
+                                    {"interleave": 2, "waves_per_simd": 8, "mix_classes": 1, "class_kernels": 1}])
 def test_mixed_views_batch_bit_exact(c3, tuning):
     """64 views (dda / svo-df / esvo; W x H log-uniform in 256..1920 x
     256..1080; orbit cameras at distance 0.5..3) over three resident volumes
@@ -113,7 +114,8 @@ def test_mixed_views_batch_bit_exact(c3, tuning):
     b = ctx.batch()
     if tuning:
         tw = b.tuning()
-        b.set_tuning(tw["tile_w"], tw["tiles_per_grab"], tuning["waves_per_simd"], tw["class_kernels"],
+        b.set_tuning(tw["tile_w"], tw["tiles_per_grab"], tuning["waves_per_simd"],
+                     tuning.get("class_kernels", tw["class_kernels"]),
                      tuning["interleave"], tuning["mix_classes"])
     b.set_views(views)
     b.render()

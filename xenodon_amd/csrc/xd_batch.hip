@@ -35,14 +35,18 @@ __global__ void k_batch_reset(uint32_t* counters) {
     if (threadIdx.x < kClasses * kQueues) counters[threadIdx.x] = 0;
 }
 
-template <int TRAV>
+// PIPE: the DDA loads the next steps' macro words while it gathers the
+// current ones (more registers; used by the DDA class kernel, whose register
+// budget is its own, not by the all-class kernel).
+template <int TRAV, bool PIPE = (TRAV == XD_DDA)>
 __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds);
 
 // Any traversal: the kernel for all classes at once dispatches on the view.
 template <>
-__device__ __forceinline__ void shade_tile<kClasses>(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds) {
+__device__ __forceinline__ void shade_tile<kClasses, false>(const BatchView& V, uint32_t tile, uint32_t lane,
+                                                            uint32_t* lds) {
     switch (V.traversal) {
-    case XD_DDA: shade_tile<XD_DDA>(V, tile, lane, lds); break;
+    case XD_DDA: shade_tile<XD_DDA, false>(V, tile, lane, lds); break;
     case XD_SVO_NAIVE: shade_tile<XD_SVO_NAIVE>(V, tile, lane, lds); break;
     case XD_ESVO: shade_tile<XD_ESVO>(V, tile, lane, lds); break;
     case XD_SVO_DF: shade_tile<XD_SVO_DF>(V, tile, lane, lds); break;
@@ -50,7 +54,7 @@ __device__ __forceinline__ void shade_tile<kClasses>(const BatchView& V, uint32_
     }
 }
 
-template <int TRAV>
+template <int TRAV, bool PIPE>
 __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds) {
     const uint32_t sh = V.tw_shift;
     const uint32_t tx = tile % V.tiles_x, ty = tile / V.tiles_x;
@@ -66,8 +70,8 @@ __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, ui
         const float side = max_elem(mk((float)p.dims[0], (float)p.dims[1], (float)p.dims[2]));
         const f3 ro = muls(p.cam.translation, side);
         coeff = emission_coeff(p.view, rd) / side;
-        c = p.wide ? dda_trace<false, true>(p, p.macro, ro, rd, words)
-                   : dda_trace<false, false>(p, p.macro, ro, rd, words);
+        c = p.wide ? dda_trace<false, true, PIPE>(p, p.macro, ro, rd, words)
+                   : dda_trace<false, false, PIPE>(p, p.macro, ro, rd, words);
     } else {
         c = p.cw ? svo_pixel<TRAV, false, true>(p, rd, lds, nullptr, words)
                  : svo_pixel<TRAV, false, false>(p, rd, lds, nullptr, words);

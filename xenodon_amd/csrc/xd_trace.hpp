@@ -59,7 +59,14 @@ __device__ __forceinline__ int32_t f2i_sat(float x) {
 // fetched only inside macro cells whose bitmask bit (staged in LDS) says they
 // hold a non-zero RGB; every skipped texel is zero and would add exactly +0.
 // ---------------------------------------------------------------------------
-template <bool COUNT, bool WIDE>
+// With PIPE the bitmask is read from device memory (for the persistent batch
+// kernels, whose waves serve views of different volumes and stage nothing in
+// LDS), and the macro words of the next batch of four steps are loaded while
+// the current batch's texels are gathered: a batch waits on one memory round
+// trip instead of two in a row.  The steps, the texels fetched and the order
+// of the sum are unchanged.  Without PIPE the bitmask may be in LDS or in
+// device memory.
+template <bool COUNT, bool WIDE, bool PIPE = false>
 __device__ __forceinline__ f3 dda_trace(const RenderParams& p, const uint32_t* __restrict__ lds_macro, f3 ro, f3 rd,
                                         uint64_t& words) {
     f3 rrd = divv(splat(1.0f), rd);
@@ -98,6 +105,71 @@ __device__ __forceinline__ f3 dda_trace(const RenderParams& p, const uint32_t* _
     const uint32_t nbx = p.nb[0], nby = p.nb[1];
     const uint32_t ms = p.macro_shift, mnx = p.mn[0], mny = p.mn[1];
     bool live = true;
+    if (PIPE) {
+        typedef typename std::conditional<WIDE, uint64_t, uint32_t>::type Addr;
+        const __attribute__((address_space(1))) uint32_t* gmacro =
+            (const __attribute__((address_space(1))) uint32_t*)lds_macro;
+        // the next four steps: texel addresses, lengths, and the macro word and
+        // bit of each step's cell (word 0 outside the grid)
+        struct Steps {
+            Addr addr[4];
+            float dts[4];
+            uint32_t word[4];
+            uint32_t bits;
+            uint32_t n;
+        };
+        auto advance = [&](Steps& S) {
+            S.bits = 0;
+            S.n = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                S.addr[j] = 0;
+                S.dts[j] = 0.0f;
+                S.word[j] = 0;
+                if (live && t < limit && ++it <= p.cap) {
+                    bool mx = sd.x <= gmin(sd.y, sd.z);
+                    bool my = sd.y <= gmin(sd.z, sd.x);
+                    bool mz = sd.z <= gmin(sd.x, sd.y);
+                    float t0 = min_elem(sd);
+                    if ((uint32_t)px < X && (uint32_t)py < Y && (uint32_t)pz < Z) {
+                        const uint32_t ux = (uint32_t)px, uy = (uint32_t)py, uz = (uint32_t)pz;
+                        const uint32_t bit = ((uz >> ms) * mny + (uy >> ms)) * mnx + (ux >> ms);
+                        S.word[j] = gmacro[bit >> 5];
+                        S.bits |= (bit & 31u) << (5 * j);
+                        const uint32_t in_brick = (ux & 3u) + ((uy & 3u) << 2) + ((uz & 3u) << 4);
+                        if (WIDE)
+                            S.addr[j] = (((uint64_t)(uz >> 2) * nby + (uy >> 2)) * nbx + (ux >> 2)) * 64u + in_brick;
+                        else
+                            S.addr[j] = (uint32_t)(((((uz >> 2) * nby + (uy >> 2)) * nbx + (ux >> 2)) << 6) + in_brick);
+                    }
+                    S.dts[j] = t0 - t;
+                    t = t0;
+                    sd = add(sd, mk(mx ? t_delta.x : 0.0f, my ? t_delta.y : 0.0f, mz ? t_delta.z : 0.0f));
+                    px += mx ? sx : 0;
+                    py += my ? sy : 0;
+                    pz += mz ? sz : 0;
+                    ++S.n;
+                } else {
+                    live = false;
+                }
+            }
+        };
+        Steps cur;
+        advance(cur);
+        while (cur.n) {
+            uint32_t tex[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                tex[j] = ((cur.word[j] >> ((cur.bits >> (5 * j)) & 31u)) & 1u) ? p.bricks[cur.addr[j]] : 0u;
+            Steps nxt;
+            advance(nxt);  // its macro loads are in flight with this batch's gathers
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (tex[j] & 0xFFFFFFu) total = add(total, muls(unorm_rgb(tex[j]), cur.dts[j]));
+            cur = nxt;
+        }
+        return total;
+    }
     while (live) {
         constexpr int B = 4;
         typename std::conditional<WIDE, uint64_t, uint32_t>::type addr[B];
