@@ -196,7 +196,9 @@ typedef struct {
  * (1..8), the register budget of the persistent kernels (0 = compiler's
  * choice, 8 = capped for 8 waves per SIMD), and whether one persistent kernel
  * drains every traversal's tiles (class_kernels = 0) or each traversal gets
- * its own kernel, with registers and LDS of that traversal only (1), and the
+ * its own kernel, with registers and LDS of that traversal only, run in turn
+ * (1; the DDA kernel then also loads the next steps' macro-cell words while
+ * the current texels are gathered), and the
  * tile order: 0 = view by view, rows of tiles; 1 = views of the same volume,
  * traversal and size interleaved tile by tile, so that the same screen tile
  * of, e.g., consecutive camera frames is traced at the same time through the
