@@ -121,3 +121,37 @@ def test_batch_rejects_bad_views(env):
     with pytest.raises(abi.XdError):
         b.set_views([(vol, abi.ESVO, abi.make_camera((0, 0, 1), (0, 1, 0), (0.5, 0.5, -2)),
                       abi.make_uniforms(8, 8, md), ctx.frame(9, 8))])
+
+
+@pytest.mark.parametrize("classes,waves", [(1, 8), (1, 0), (0, 8)])
+def test_dda_batch_edge_views(env, classes, waves):
+    """DDA in the batch kernels on a ragged grid (29 x 50 x 37: bricks cut by
+    the grid's faces): eyes inside the grid, axis-aligned views (rays with
+    zero direction components), and a grazing oblique view; every frame
+    equals the plain render."""
+    ctx, _ = env
+    grid = np.ascontiguousarray(H.blocky_grid(64, 4, seed=43)[:37, :50, :29])
+    vol = ctx.grid_volume(grid)
+    md = [29, 50, 37]
+    cams = [H.look_at((0.5, 0.5, 0.5), (1.5, 0.5, 0.5)),               # inside, along +x
+            H.look_at((0.3, 0.6, 0.45), (0.3, 0.6, -1.0)),             # inside, along -z
+            H.look_at((0.5, 0.5, 2.5), (0.5, 0.5, 0.0)),               # outside, along -z
+            H.look_at((-0.4, 0.02, 0.5), (1.2, 0.03, 0.52)),           # grazing the y = 0 face
+            H.look_at((1.7, 1.3, -0.6), (0.45, 0.55, 0.5))]            # oblique
+    views = [(vol, abi.DDA, abi.make_camera(f, up, eye), abi.make_uniforms(97, 61, md, emission=2.5), ctx.frame(97, 61))
+             for f, up, eye in cams]
+    b = ctx.batch()
+    tw = b.tuning()
+    b.set_tuning(tw["tile_w"], tw["tiles_per_grab"], waves, classes, tw["interleave"], tw["mix_classes"])
+    b.set_views(views)
+    b.render()
+    lit = 0
+    for v, t, cam, uni, fr in views:
+        got = fr.download()
+        ref, _ = ctx.render_image(v, t, cam, uni)
+        assert np.array_equal(got, ref)
+        lit += int(got[..., :3].any())
+        fr.free()
+    assert lit >= 4
+    b.free()
+    vol.free()
