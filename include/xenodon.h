@@ -198,7 +198,8 @@ typedef struct {
  * drains every traversal's tiles (class_kernels = 0) or each traversal gets
  * its own kernel, with registers and LDS of that traversal only, run in turn
  * (1; the DDA kernel then also loads the next steps' macro-cell words while
- * the current texels are gathered), and the
+ * the current texels are gathered; a batch of one traversal always runs that
+ * traversal's own kernel), and the
  * tile order: 0 = view by view, rows of tiles; 1 = views of the same volume,
  * traversal and size interleaved tile by tile, so that the same screen tile
  * of, e.g., consecutive camera frames is traced at the same time through the

@@ -312,9 +312,16 @@ int upload_tables(xd_batch* b, BatchClasses* need) {
                         for (uint32_t i : G.views)
                             q[(uint32_t)G.key[1] * kQueues + band].push_back((i << 20) | (ty * G.tiles_x + tx));
         }
+    // A batch of one traversal runs that traversal's own kernel whatever the
+    // tuning says: with one class there is nothing to mix, and the class
+    // kernel has that traversal's register budget (and, for DDA, the loop
+    // that loads the next steps' macro words during the current gathers).
+    uint32_t present = 0;
+    for (uint32_t c = 0; c < kClasses; ++c) present += cls.grid[c] ? 1u : 0u;
+    const bool per_class = b->tuning.class_kernels || present == 1;
     // one LDS layout per kernel: every wave's stack region is sized for the
     // deepest tree among the views that kernel drains
-    if (!b->tuning.class_kernels) {
+    if (!per_class) {
         uint32_t all = 0, any = 0;
         for (uint32_t c = 0; c < kClasses; ++c) {
             all = std::max(all, cls.slots[c]);
@@ -325,7 +332,7 @@ int upload_tables(xd_batch* b, BatchClasses* need) {
         cls.slots[kClasses] = all;
     }
     for (uint32_t i = 0; i < n; ++i)
-        bv[i].p.lds_slots = std::max<uint32_t>(cls.slots[b->tuning.class_kernels ? bv[i].traversal : kClasses], 1);
+        bv[i].p.lds_slots = std::max<uint32_t>(cls.slots[per_class ? bv[i].traversal : kClasses], 1);
     uint32_t qbeg[kClasses * kQueues + 1];
     uint32_t total = 0;
     for (uint32_t k = 0; k < kClasses * kQueues; ++k) {
