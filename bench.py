@@ -564,8 +564,10 @@ def main():
                         "l2_hit": k.get("l2_hit"), "valu_lane_utilisation": k.get("valu_lane_utilisation")}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                # a batch of one traversal always runs that traversal's own kernel
                 "kernel": "xd::" + ("k_batch_w8" if tuning["waves_per_simd"] == 8 else "k_batch")
-                          + ("<0..4> (persistent, one per traversal)" if tuning["class_kernels"]
+                          + (f"<{','.join(str(abi.TRAVERSALS[t]) for t in travs)}> (persistent, one per traversal)"
+                             if tuning["class_kernels"] or len(travs) == 1
                              else "<5> (persistent, all five traversals)"),
                 "algorithmic_bytes_per_launch": round(float(np.mean(batch_algo))),
                 "mean_launch_ms": round(mean_batch_ms, 4), **measured,
