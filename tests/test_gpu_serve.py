@@ -97,10 +97,12 @@ def test_served_frames_match_plain(env, class_kernels):
     srv.free()
 
 
-def test_result_is_the_same_alone_and_in_the_mix(env):
+@pytest.mark.parametrize("name,t", [("shells_svo", abi.SVO_DF), ("shells_grid", abi.DDA)])
+def test_result_is_the_same_alone_and_in_the_mix(env, name, t):
+    """Alone, the request's steps hold one traversal and run its own kernel;
+    in the mix they run the all-traversal kernel."""
     ctx, data, plain = env
-    probe = ("shells_svo", abi.SVO_DF, H.orbit_cameras(1, 1.7, seed=99)[0], 333, 217, 3.0)
-    name, t, cam, W, Hh, e = probe
+    cam, W, Hh, e = H.orbit_cameras(1, 1.7, seed=99)[0], 333, 217, 3.0
     # alone
     srv = ctx.server(tiles_per_launch=256)
     ids = _load_all(srv, data)

@@ -269,12 +269,17 @@ bool launch_step(xd_server* s, Step& S, std::string* err) {
         base += count[c];
         all_slots = std::max(all_slots, cls.slots[c]);
     }
-    if (!s->class_kernels) {  // one kernel drains every class
+    // a step of one traversal runs that traversal's own kernel (nothing to
+    // mix, and the traversal's own register budget)
+    uint32_t classes = 0;
+    for (uint32_t c = 0; c < kClasses; ++c) classes += count[c] ? 1u : 0u;
+    const bool per_class = s->class_kernels || classes == 1;
+    if (!per_class) {  // one kernel drains every class
         for (uint32_t c = 0; c < kClasses; ++c) cls.slots[c] = 0;
         cls.slots[kClasses] = all_slots;
     }
     for (uint32_t c = 0; c <= kClasses; ++c) {
-        const bool present = s->class_kernels ? (c < kClasses && count[c]) : c == kClasses;
+        const bool present = per_class ? (c < kClasses && count[c]) : c == kClasses;
         if (!present) continue;
         cls.grid[c] = grid_for(s, (int)c, cls.slots[c]);
         if (!cls.grid[c]) {
@@ -283,7 +288,7 @@ bool launch_step(xd_server* s, Step& S, std::string* err) {
         }
     }
     for (size_t j = 0; j < S.parts.size(); ++j)
-        views[j].p.lds_slots = std::max<uint32_t>(cls.slots[s->class_kernels ? views[j].traversal : kClasses], 1);
+        views[j].p.lds_slots = std::max<uint32_t>(cls.slots[per_class ? views[j].traversal : kClasses], 1);
     qbeg[kClasses * kQueues] = base;
     const size_t bytes = s->tiles_off + (size_t)n * sizeof(uint32_t);
     hipError_t e;
