@@ -264,7 +264,9 @@ typedef struct {
     uint32_t tiles_per_launch;   /* wave tiles per scheduling step (0 = 65536) */
     uint32_t launches_in_flight; /* 1..4 (0 = 2) */
     uint32_t readback_threads;   /* host threads copying finished frames out (1..16; 0 = 4) */
-    xd_batch_tuning tuning;      /* tile shapes etc.; zero-filled = defaults */
+    xd_batch_tuning tuning;      /* tile shapes etc.; zero-filled = defaults.  A
+                                    step whose tiles are all of one traversal
+                                    runs that traversal's own kernel */
 } xd_server_config;
 
 typedef struct {
