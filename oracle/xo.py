@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, always as the checker.  The product
 (xenodon_amd) never imports this module.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -58,32 +59,59 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def _load(path):
+    L = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    L.xo_render.argtypes = [ctypes.c_int, P(Volume), P(Camera), P(Uniforms), ctypes.c_int, ctypes.c_int,
+                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.xo_trace_ray.argtypes = [ctypes.c_int, P(Volume), P(ctypes.c_double), P(ctypes.c_double), ctypes.c_int,
+                               P(ctypes.c_double), P(ctypes.c_uint64)]
+    L.xo_ray_dir.argtypes = [P(Camera), P(Uniforms), ctypes.c_uint32, ctypes.c_uint32, P(ctypes.c_float)]
+    L.xo_iteration_cap.argtypes = [ctypes.c_int, P(Volume), P(ctypes.c_uint32)]
+    L.xo_iteration_cap.restype = ctypes.c_uint64
+    L.xo_build_octree.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_int, P(ctypes.c_void_p), P(BuildStats)]
+    L.xo_generate_ropes.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
+    L.xo_octree_find.argtypes = [ctypes.c_void_p, ctypes.c_uint64, P(ctypes.c_uint64), ctypes.c_uint64]
+    L.xo_octree_find.restype = ctypes.c_uint64
+    L.xo_vol_scan.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_uint64), P(ctypes.c_uint64),
+                              P(ctypes.c_uint32), P(ctypes.c_uint32)]
+    L.xo_stddev_scan.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_uint64), P(ctypes.c_uint64),
+                                 P(ctypes.c_uint32), P(ctypes.c_double)]
+    L.xo_free.argtypes = [ctypes.c_void_p]
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        P = ctypes.POINTER
-        L.xo_render.argtypes = [ctypes.c_int, P(Volume), P(Camera), P(Uniforms), ctypes.c_int, ctypes.c_int,
-                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
-        L.xo_trace_ray.argtypes = [ctypes.c_int, P(Volume), P(ctypes.c_double), P(ctypes.c_double), ctypes.c_int,
-                                   P(ctypes.c_double), P(ctypes.c_uint64)]
-        L.xo_ray_dir.argtypes = [P(Camera), P(Uniforms), ctypes.c_uint32, ctypes.c_uint32, P(ctypes.c_float)]
-        L.xo_iteration_cap.argtypes = [ctypes.c_int, P(Volume), P(ctypes.c_uint32)]
-        L.xo_iteration_cap.restype = ctypes.c_uint64
-        L.xo_build_octree.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), ctypes.c_int, ctypes.c_int,
-                                      ctypes.c_double, ctypes.c_int, P(ctypes.c_void_p), P(BuildStats)]
-        L.xo_generate_ropes.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64]
-        L.xo_octree_find.argtypes = [ctypes.c_void_p, ctypes.c_uint64, P(ctypes.c_uint64), ctypes.c_uint64]
-        L.xo_octree_find.restype = ctypes.c_uint64
-        L.xo_vol_scan.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_uint64), P(ctypes.c_uint64),
-                                  P(ctypes.c_uint32), P(ctypes.c_uint32)]
-        L.xo_stddev_scan.argtypes = [ctypes.c_void_p, P(ctypes.c_uint64), P(ctypes.c_uint64), P(ctypes.c_uint64),
-                                     P(ctypes.c_uint32), P(ctypes.c_double)]
-        L.xo_free.argtypes = [ctypes.c_void_p]
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def build_mutant(mutant, out_dir):
+    """Builds the oracle with one deliberate mistake planted (-DXO_MUTANT=k,
+    the list is in xo_trace_body.h) into out_dir and returns the library path.
+    Used only by the tests that show each pin of the oracle fails under the
+    mistake it is meant to catch."""
+    path = os.path.join(out_dir, f"liboracle_m{int(mutant)}.so")
+    subprocess.run(["make", "-s", "-C", HERE, f"MUTANT={int(mutant)}", f"OUT={path}", path], check=True)
+    return path
+
+
+@contextlib.contextmanager
+def use_library(path):
+    """Routes every call of this module to the library at `path` (a mutant
+    build) for the duration of the block."""
+    global _lib
+    saved = lib()
+    _lib = _load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def make_camera(forward, up, translation):

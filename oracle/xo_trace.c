@@ -2,9 +2,19 @@
  * Oracle traversal entry points.  TEST INFRASTRUCTURE ONLY (see xo.h).
  * Build with: gcc -O2 -std=c11 -ffp-contract=off -fno-fast-math -fopenmp
  *
- * Parity status (tests/test_oracle_trace.py):
- *   xo_ray_dir, emission coefficient   pinned: uniform-cube closed form
- *                                      (pixel = c*e*chord) at every pixel
+ * Parity status (tests/test_oracle_trace.py, tests/test_oracle_projection.py,
+ * and tests/test_oracle_mutants.py, which plants one mistake at a time and
+ * shows the pins below fail under it):
+ *   ray() / xo_ray_dir                 pinned by an independent pinhole model
+ *                                      (tests/helpers.py): chord field from an
+ *                                      eye inside a uniform cube, centre pixel
+ *                                      = forward, edge pixel at atan(1/2),
+ *                                      lit pixels of one voxel = its forward
+ *                                      projection (orientation, pixel corner)
+ *   emission coefficient, voxel_ratio  pinned: pixel = c*e*(camera-space chord
+ *                                      in [0, voxel_ratio])
+ *   unorm8 store                       pinned: Vulkan rule (clamp, *255, round
+ *                                      to nearest) on every pixel
  *   DDA (dda.comp:13-53)               pinned: empty/miss closed forms, uniform
  *                                      cube, FP64 brute-force voxel-box integral
  *                                      on 2^3..16^3 and ragged 3x7x5 grids
@@ -12,7 +22,9 @@
  *                                      brute-force integrals on the lossless
  *                                      tree, cross-traversal invariant
  *   FP64 twin                          pinned: agrees with FP32 to 2e-5 rel.
- *   word counters                      pinned: hand-derived known answers
+ *   word counters                      pinned: hand-derived known answers for
+ *                                      all five traversals (a 16-voxel DDA
+ *                                      column; a hand-built two-level tree)
  *   xo_iteration_cap                   pinned: never reached on well-formed input
  */
 #include "xo.h"
@@ -109,14 +121,22 @@ static inline uint8_t xo_unorm8(float v) {
         return 0;
     if (v >= 1.0f)
         return 255;
+#if XO_MUTANT == 6
+    return (uint8_t)(v * 255.0f);
+#else
     return (uint8_t)rintf(v * 255.0f);
+#endif
 }
 static inline uint8_t xo_unorm8d(double v) {
     if (!(v > 0.0))
         return 0;
     if (v >= 1.0)
         return 255;
+#if XO_MUTANT == 6
+    return (uint8_t)(v * 255.0);
+#else
     return (uint8_t)rint(v * 255.0);
+#endif
 }
 
 static int xo_check(int traversal, const xo_volume* vol) {

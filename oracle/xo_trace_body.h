@@ -14,7 +14,24 @@
  *   unorm8 -> float: c / 255 (correctly rounded)
  * Expressions are evaluated left to right as written in the shader, one
  * rounding per operation.
+ *
+ * XO_MUTANT=k (0, the default, is the faithful oracle) plants one deliberate
+ * mistake, so that the tests can show each pin of the oracle fails under the
+ * mistake it is meant to catch (tests/test_oracle_mutants.py):
+ *   1  right = cross(dir, up): the image mirrored left-right
+ *   2  uv at the pixel centre (+0.5) instead of its corner
+ *   3  aspect applied as uv.x *= W/H instead of uv.y *= H/W
+ *   4  up = cross(dir, right): the image upside down
+ *   5  emission coefficient with voxel_ratio instead of voxel_ratio^2
+ *   6  the unorm8 store truncates instead of rounding to nearest
+ *   7  ray() does not divide the direction by voxel_ratio
+ *   8  the svo-df pop does not count its is_leaf_depth word
+ *   9  esvo does not count the leaf colour word
+ *   10 horizontal field of view 60 degrees instead of 2*atan(1/2)
  */
+#ifndef XO_MUTANT
+#define XO_MUTANT 0
+#endif
 
 #define XO_CAT2(a, b) a##_##b
 #define XO_CAT(a, b) XO_CAT2(a, b)
@@ -75,20 +92,44 @@ static inline V3 F(adjust_ray)(V3 rd) {
 static V3 F(ray)(const xo_camera* cam, const xo_uniforms* u, REAL uvx, REAL uvy) {
     uvx = uvx - (REAL)0.5;
     uvy = uvy - (REAL)0.5;
+#if XO_MUTANT == 3
+    uvx = uvx * ((REAL)u->disp_extent[0] / (REAL)u->disp_extent[1]);
+#else
     uvy = uvy * ((REAL)u->disp_extent[1] / (REAL)u->disp_extent[0]);
+#endif
+#if XO_MUTANT == 10
+    uvx = uvx * (REAL)1.1547005383792515; /* tan(30 deg) / (1/2) */
+    uvy = uvy * (REAL)1.1547005383792515;
+#endif
     V3 dir = F(vload)(cam->forward);
     V3 up = F(vload)(cam->up);
+#if XO_MUTANT == 1
+    V3 right = F(normalize)(F(cross)(dir, up));
+#else
     V3 right = F(normalize)(F(cross)(up, dir));
+#endif
+#if XO_MUTANT == 4
+    up = F(normalize)(F(cross)(dir, right));
+#else
     up = F(normalize)(F(cross)(right, dir));
+#endif
     V3 rd = F(normalize)(F(add)(F(add)(F(smul)(uvx, right), F(smul)(uvy, up)), dir));
+#if XO_MUTANT == 7
+    return F(adjust_ray)(F(normalize)(rd));
+#else
     return F(adjust_ray)(F(normalize)(F(divv)(rd, F(vload)(u->voxel_ratio))));
+#endif
 }
 
 /* common.glsl:68-72 voxel_emission_coeff */
 static REAL F(emission_coeff)(const xo_uniforms* u, V3 rd) {
     V3 rd2 = F(mul)(rd, rd);
     V3 ratio = F(vload)(u->voxel_ratio);
+#if XO_MUTANT == 5
+    V3 dim2 = ratio;
+#else
     V3 dim2 = F(mul)(ratio, ratio);
+#endif
     return (REAL)u->emission_coeff * SQRT(F(dot)(rd2, dim2) / F(dot)(rd2, F(splat)(1)));
 }
 
@@ -268,7 +309,12 @@ static V3 F(df_trace)(const xo_volume* v, V3 ro, V3 rd, uint64_t cap, uint64_t* 
                 break;
             node = node_stack[sp];
             child_idx = child_index_stack[sp];
+#if XO_MUTANT == 8
+            uint64_t uncounted = 0;
+            side = LDEXP((REAL)1, -(int)(F(leafdepth_of)(v, node, &uncounted) & XO_DEPTH_MASK)) * (REAL)0.5;
+#else
             side = LDEXP((REAL)1, -(int)(F(leafdepth_of)(v, node, w) & XO_DEPTH_MASK)) * (REAL)0.5;
+#endif
         }
         REAL s2 = side * (REAL)2.0;
         pos = F(sub)(pos, F(mk)(F(gmod)(pos.x, s2), F(gmod)(pos.y, s2), F(gmod)(pos.z, s2)));
@@ -488,7 +534,12 @@ static V3 F(esvo_trace)(const xo_volume* v, V3 ro, V3 rd, uint64_t cap, uint64_t
             if (t_min <= tv_max) {
                 uint32_t child = F(child_of)(v, parent, idx ^ octant_mask, w);
                 if (F(leafdepth_of)(v, child, w) >= XO_LEAF_MASK) {
+#if XO_MUTANT == 9
+                    uint64_t uncounted = 0;
+                    V3 color = F(unorm_rgb)(F(color_of)(v, child, &uncounted));
+#else
                     V3 color = F(unorm_rgb)(F(color_of)(v, child, w));
+#endif
                     total = F(add)(total, F(muls)(color, tv_max - t_min));
                 } else {
                     /* PUSH (scale wraps below zero only past depth 22; the loop then ends) */
@@ -537,8 +588,13 @@ static V3 F(pixel)(int traversal, const xo_volume* vol, const xo_camera* cam, co
                    uint32_t iy, uint64_t cap, uint64_t* words) {
     int32_t pxl = u->out_offset[0] + (int32_t)ix;
     int32_t pyl = u->out_offset[1] + (int32_t)iy;
+#if XO_MUTANT == 2
+    REAL uvx = ((REAL)(pxl - u->disp_offset[0]) + (REAL)0.5) / (REAL)u->disp_extent[0];
+    REAL uvy = ((REAL)(pyl - u->disp_offset[1]) + (REAL)0.5) / (REAL)u->disp_extent[1];
+#else
     REAL uvx = (REAL)(pxl - u->disp_offset[0]) / (REAL)u->disp_extent[0];
     REAL uvy = (REAL)(pyl - u->disp_offset[1]) / (REAL)u->disp_extent[1];
+#endif
     V3 rd = F(ray)(cam, u, uvx, uvy);
     V3 tr = F(vload)(cam->translation);
     if (traversal == XO_DDA) {

@@ -112,3 +112,54 @@ def dda_shifted_integral(grid, ro, rd):
 def unorm8(x):
     x = np.nan_to_num(np.asarray(x, np.float64), nan=0.0)
     return np.rint(np.clip(x, 0.0, 1.0) * 255.0).astype(np.int32)
+
+
+# ---------------------------------------------------------------------------
+# The reference camera stated geometrically (resources/common.glsl:45-56 and
+# resources/help/render.txt:33-41), written independently of the oracle's
+# ray() so that it can pin it:
+#   * a pinhole at the eye, the image plane one unit ahead along `forward`
+#     and one unit wide, so the horizontal field of view is 2*atan(1/2);
+#   * square pixels, so the plane is H/W units tall;
+#   * columns run along up x forward, rows run away from `up` (row 0 is the
+#     top of the image, the side `up` points to);
+#   * pixel (i, j) is sampled at its corner, not its centre:
+#     plane point (i/W - 1/2, (H/2 - j)/W);
+#   * the volume spans [0, voxel_ratio] in camera space.
+# ---------------------------------------------------------------------------
+
+
+def camera_basis(forward, up):
+    f = np.asarray(forward, np.float64)
+    f = f / np.linalg.norm(f)
+    u = np.asarray(up, np.float64)
+    u = u / np.linalg.norm(u)
+    r = np.cross(u, f)
+    return f, r / np.linalg.norm(r), u
+
+
+def pinhole_dir(forward, up, W, H, i, j):
+    """Unit direction of pixel (i, j) of a W x H display (FP64)."""
+    f, r, u = camera_basis(forward, up)
+    d = f + (i / W - 0.5) * r + ((H / 2.0 - j) / W) * u
+    return d / np.linalg.norm(d)
+
+
+def project_point(forward, up, eye, W, H, p):
+    """Continuous pixel coordinates (i, j) of world point p: the inverse of
+    pinhole_dir, so pixel (i, j) looks exactly at it."""
+    f, r, u = camera_basis(forward, up)
+    v = np.asarray(p, np.float64) - np.asarray(eye, np.float64)
+    depth = v @ f
+    x = (v @ r) / depth
+    y = (v @ u) / depth
+    return (x + 0.5) * W, H / 2.0 - y * W
+
+
+def exit_distance(eye, d, box_hi):
+    """Distance from `eye` inside the box [0, box_hi] to the box boundary
+    along unit direction d (FP64)."""
+    eye = np.asarray(eye, np.float64)
+    hi = np.asarray(box_hi, np.float64)
+    t = [((hi[k] if d[k] > 0 else 0.0) - eye[k]) / d[k] for k in range(3) if d[k] != 0]
+    return min(t)
