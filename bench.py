@@ -108,22 +108,54 @@ def read_cameras(path):
     return [(r[0:3], r[3:6], r[6:9]) for r in v]
 
 
-def cpu_baseline(grid, nodes, side, cams, frames, threads, budget_s=30.0, width=WIDTH, height=HEIGHT,
+def host_cpu():
+    """The host's CPU model, its logical CPUs, the CPUs this process may run
+    on, and the threads the CPU baseline uses: all of those CPUs, or the
+    job's share where OMP_NUM_THREADS sets one (a GPU box allots each GPU
+    job a share of its cores that way)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(usable, share) if share > 0 else usable
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable,
+            "omp_num_threads": share or None, "threads": threads}
+
+
+def spread_order(n):
+    """0..n-1 in van der Corput order: every prefix is spread over the range."""
+    bits = max(1, (n - 1).bit_length())
+    keys = []
+    for i in range(1 << bits):
+        r = int(format(i, f"0{bits}b")[::-1], 2)
+        if r < n:
+            keys.append(r)
+    return keys
+
+
+def cpu_baseline(grid, nodes, side, cams, frame_ids, host, budget_s=30.0, width=WIDTH, height=HEIGHT,
                  emission=EMISSION, travs=TRAV_ORDER):
-    """The oracle on the host cores: full frames of the same path (frames
-    evenly spaced over it), every traversal of the workload per frame,
-    stopping after the frame that crosses budget_s seconds."""
+    """The oracle on the host's cores: full frames of the timed steps' camera
+    frames (frame_ids, in an order whose every prefix is spread over them),
+    every traversal of the workload per frame, stopping after the frame that
+    crosses budget_s seconds."""
     from oracle import xo  # the checker, timed here only as the CPU baseline
 
+    threads = host["threads"]
     og = xo.OracleVolume(grid=grid)
     osv = xo.OracleVolume(nodes=nodes, side=side) if any(n != "dda" for n in travs) else None
     rays = 0
     secs = 0.0
     per_rays = {n: 0 for n in travs}
     per_secs = {n: 0.0 for n in travs}
-    picked = [int(round(i * len(cams) / frames)) % len(cams) for i in range(frames)]
     done = []
-    for k in picked:
+    for k in frame_ids:
         f, up, eye = cams[k]
         for name in travs:
             t = abi.TRAVERSALS[name]
@@ -139,14 +171,19 @@ def cpu_baseline(grid, nodes, side, cams, frames, threads, budget_s=30.0, width=
         done.append(k)
         if secs > budget_s:
             break
+    value = rays / secs / 1e6
     return {
-        "value": round(rays / secs / 1e6, 3),
+        "value": round(value, 3),
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle FP32 restatement on {threads} threads: camera frames {done} of the path at "
-                  f"{width}x{height}, {len(travs)} traversal{'s' if len(travs) > 1 else ''} each ({rays} rays, "
-                  f"{secs:.1f} s)",
+        "sample": f"oracle FP32 restatement on {threads} threads of {host['cpu_model']} (host: "
+                  f"{host['logical_cpus']} logical CPUs, {host['usable_cpus']} usable by this process"
+                  + (f", job share OMP_NUM_THREADS={host['omp_num_threads']}" if host["omp_num_threads"] else "")
+                  + f"): camera frames {done} of the timed steps' frames at {width}x{height}, {len(travs)} "
+                  f"traversal{'s' if len(travs) > 1 else ''} each ({rays} rays, {secs:.1f} s)",
+        "per_core_mrays": round(value / threads, 4),
+        "host": host,
         "per_traversal_mrays": {n: round(per_rays[n] / per_secs[n] / 1e6, 3) for n in travs if per_secs[n]},
     }
 
@@ -273,7 +310,9 @@ def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plai
            "p50_device_ms": round(float(np.percentile(dev, 50)), 3),
            "p99_device_ms": round(float(np.percentile(dev, 99)), 3),
            "mean_rays_per_request": round(rays / n), "launches": st["launches"] - st0["launches"],
-           "device_busy_frac": round((st["busy_ms"] - st0["busy_ms"]) / wall, 3)}
+           # summed launch durations over the wall time: launches overlap on
+           # the device, so this can exceed 1 (it is not a busy fraction)
+           "launch_ms_sum_over_wall": round((st["busy_ms"] - st0["busy_ms"]) / wall, 3)}
     if server is None:
         res["max_requests_per_launch"] = st["max_requests_per_launch"]
     if plain:
@@ -303,7 +342,10 @@ def main():
                     help="camera frames timed, each with every traversal of the workload (default: 150; 1500 for "
                          "shells128, its 10-frame orbit cycled)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--frame-stride", type=int, default=1, help="step k renders camera frame k*stride")
+    ap.add_argument("--frame-order", choices=["spread", "consecutive"], default=None,
+                    help="spread (default for camera.txt): step k of n renders frame round(k * 150 / n), so any "
+                         "number of steps samples the orbit, the zoom and the frames from inside the volume; "
+                         "consecutive: step k renders frame k (the default for the cycled 10-frame orbit)")
     ap.add_argument("--batch-frames", type=int, default=None,
                     help="camera frames per batched launch (default: 10; 150 for shells128's 256x256 frames)")
     ap.add_argument("--per-traversal-groups", type=int, default=3,
@@ -402,8 +444,16 @@ def main():
     def uni(md):
         return abi.make_uniforms(width, height, md, emission=emission)
 
+    if args.frame_order is None:
+        args.frame_order = "spread" if wl["cameras"] == "path" else "consecutive"
+
+    def frame_index(k):
+        if args.frame_order == "spread":
+            return workloads.path_frame(k, args.steps, len(cams))
+        return k % len(cams)
+
     def frame_of(k):
-        return cams[(k * args.frame_stride) % len(cams)]
+        return cams[frame_index(k)]
 
     def step(k, ms_acc):
         f, up, eye = frame_of(k)
@@ -548,20 +598,33 @@ def main():
         per_trav_batched[name] = rays_sum / (ms_sum * 1e3)
         sub.free()
 
+    # the committed rocprofv3 summary of this same configuration (the same
+    # command line): HBM bytes per batched launch (FETCH_SIZE + WRITE_SIZE,
+    # scaled by a device-copy calibration) and that kernel's mean duration in
+    # the profiled run's kernel trace give the measured DRAM rate; its stall
+    # counters name what limits the kernel
+    run_config = {"workload": args.workload, "steps": len(groups) * B, "warmup": args.warmup,
+                  "frame_order": args.frame_order, "batch_frames": B, "traversals": travs,
+                  "tuning": {k: tuning[k] for k in sorted(tuning)}}
     traffic = None
-    measured = {}
+    measured = {"profile": None}
     profile_summary = os.path.join(ROOT, "profiles", wl["profile"])
     if os.path.exists(profile_summary):
-        # HBM bytes per launch of the same launch shape from the committed
-        # rocprofv3 counter runs; over this run's launch time they give the
-        # physically bounded rate next to the algorithmic one
         prof = json.load(open(profile_summary))
-        traffic = prof.get("hbm_bytes_per_launch", {}).get("batch")
-        k = prof.get("per_kernel", {}).get("batch", {})
-        if traffic:
-            measured = {"measured_hbm_GBs": round(traffic / (mean_batch_ms * 1e-3) / 1e9, 1),
-                        "measured_frac": round(traffic / (mean_batch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                        "l2_hit": k.get("l2_hit"), "valu_lane_utilisation": k.get("valu_lane_utilisation")}
+        if prof.get("config") == run_config:
+            traffic = prof["hbm_bytes_per_launch"]["batch"]
+            k = prof["per_kernel"]["batch"]
+            prof_ms = k["mean_launch_ms"]
+            gbs = traffic / (prof_ms * 1e-3) / 1e9
+            measured = {"profile": "profiles/" + wl["profile"], "profile_mean_launch_ms": prof_ms,
+                        "measured_hbm_GBs": round(gbs, 1), "measured_frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "l2_hit": k.get("l2_hit"), "valu_lane_utilisation": k.get("valu_lane_utilisation"),
+                        "wave_cycles": k.get("wave_cycles"), "limiter": prof.get("limiter")}
+        else:
+            measured = {"profile": f"profiles/{wl['profile']} covers another configuration ({prof.get('config')})"}
+    # The path has no matrix work, so HBM bandwidth is the roofline that
+    # bounds it; `limiter` says from the counters how far below that ceiling
+    # the kernel runs and what holds it there
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 # a batch of one traversal always runs that traversal's own kernel
@@ -664,9 +727,15 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", 0) or min(16, os.cpu_count() or 1))
-        cpu = cpu_baseline(grid, nodes, side, cams, args.cpu_frames, threads, width=width, height=height,
-                           emission=emission, travs=travs)
+        # the timed steps' own frames, in an order whose every prefix is spread
+        # over them, up to --cpu-frames of them and about 30 s of CPU work
+        step_frames = [frame_index(k) for k in range(len(groups) * B)]
+        order = []
+        for i in spread_order(len(step_frames)):
+            if step_frames[i] not in order:
+                order.append(step_frames[i])
+        cpu = cpu_baseline(grid, nodes, side, cams, order[:args.cpu_frames], host_cpu(), width=width,
+                           height=height, emission=emission, travs=travs)
 
     out = {
         "metric": "Mrays/s per traversal @1080p, batched/served vs plain; p50/p99 request latency",
@@ -681,7 +750,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": wl["data"],
-        "config": {"workload": wl["desc"],
+        "config": {"workload": wl["desc"], "frame_order": args.frame_order,
+                   "frames_timed": sorted({frame_index(k) for k in range(len(groups) * B)}),
                    "form": f"batched: persistent ray-tile queue, {B} frames x {len(travs)} traversals per "
                            "hipGraph launch", "emission_coeff": emission,
                    "saturated_pixel_fraction": saturated, "traversals": travs,
@@ -690,7 +760,9 @@ def main():
         "plain": {"value": round(plain_value, 2), "wall_s": round(plain_wall, 3), "order": args.plain_order,
                   "per_traversal_mrays": {n: round(v, 1) for n, v in per_trav.items()},
                   "per_traversal_mean_ms": {n: round(v, 4) for n, v in mean_ms.items()}},
-        "batched_per_traversal_mrays": {n: round(v, 1) for n, v in per_trav_batched.items()},
+        # each traversal alone in launches of its own: those run that
+        # traversal's class kernel, not the all-traversal kernel `value` times
+        "batched_per_traversal_class_kernel_mrays": {n: round(v, 1) for n, v in per_trav_batched.items()},
         "mixed_views": mixed,
         "served": served,
         "served_2048": served_c5,

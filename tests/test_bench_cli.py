@@ -44,8 +44,33 @@ def test_orbit_cameras():
 def test_cpu_baseline_contract():
     grid = synth.shells_128()
     nodes, st = abi.build_octree(grid, tree=abi.TREE_ROPE)
-    res = bench.cpu_baseline(grid, nodes, st["side"], bench.orbit_cameras(), frames=2, threads=2, budget_s=0.0,
+    host = dict(bench.host_cpu(), threads=2)
+    res = bench.cpu_baseline(grid, nodes, st["side"], bench.orbit_cameras(), [3, 7], host, budget_s=0.0,
                              width=32, height=24, emission=4.0, travs=["dda", "esvo"])
     assert res["unit"] == "Mrays/s" and res["kind"] == "port" and res["cores"] == 2
     assert res["value"] > 0 and set(res["per_traversal_mrays"]) == {"dda", "esvo"}
-    assert "32x24" in res["sample"] and "2 traversals" in res["sample"]
+    assert "32x24" in res["sample"] and "2 traversals" in res["sample"] and "frames [3]" in res["sample"]
+    assert host["cpu_model"] in res["sample"]
+
+
+def test_host_cpu_threads_follow_the_job_share(monkeypatch):
+    monkeypatch.setenv("OMP_NUM_THREADS", "1")
+    assert bench.host_cpu()["threads"] == 1
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    h = bench.host_cpu()
+    assert h["threads"] == h["usable_cpus"] >= 1
+
+
+def test_spread_order_prefixes_cover_the_range():
+    for n in (1, 2, 7, 20, 150):
+        o = bench.spread_order(n)
+        assert sorted(o) == list(range(n))
+    o = bench.spread_order(20)
+    assert max(o[:4]) >= 15 and min(o[:4]) == 0  # four frames already span the path
+
+
+def test_steps_sample_the_whole_camera_path():
+    from xenodon_amd import workloads
+    frames = [workloads.path_frame(k, 20, 150) for k in range(20)]
+    assert frames[0] == 0 and frames[-1] >= 140 and len(set(frames)) == 20
+    assert [workloads.path_frame(k, 150, 150) for k in range(150)] == list(range(150))

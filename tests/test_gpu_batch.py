@@ -93,7 +93,11 @@ def test_batch_reuses_graph_with_new_cameras_and_autotunes(env):
         _check_against_plain(ctx, views2)
 
 
-def test_batch_matches_oracle_on_c1(env):
+@pytest.mark.parametrize("with_octree_view", [False, True])
+def test_batch_matches_oracle_on_c1(env, with_octree_view):
+    """Four C1 dda views in one launch.  Alone they run the DDA class kernel
+    (a one-traversal batch always does); with one svo-naive view added the
+    batch runs the all-traversal kernel on the same dda views."""
     ctx, vols = env
     grid = synth.shells_128()
     ov = xo.OracleVolume(grid=grid)
@@ -103,11 +107,17 @@ def test_batch_matches_oracle_on_c1(env):
     cams = H.orbit_cameras(4, 2 * np.sqrt(3), seed=7)
     views = [(vol, abi.DDA, abi.make_camera(*c), abi.make_uniforms(256, 256, md, emission=4.0), ctx.frame(256, 256))
              for c in cams]
-    b.set_views(views)
+    extra = []
+    if with_octree_view:
+        svol, smd = vols["shells_svo"]
+        extra = [(svol, abi.SVO_NAIVE, abi.make_camera(*cams[0]), abi.make_uniforms(64, 48, smd, emission=4.0),
+                  ctx.frame(64, 48))]
+    b.set_views(views + extra)
     b.render()
     for (v, t, cam, uni, fr), c in zip(views, cams):
         ref = xo.render(xo.DDA, ov, xo.make_camera(*c), xo.make_uniforms(256, 256, md, emission=4.0))["rgba"]
         assert np.array_equal(fr.download(), ref)
+    _check_against_plain(ctx, extra)
 
 
 def test_batch_rejects_bad_views(env):
@@ -140,6 +150,13 @@ def test_dda_batch_edge_views(env, classes, waves):
             H.look_at((1.7, 1.3, -0.6), (0.45, 0.55, 0.5))]            # oblique
     views = [(vol, abi.DDA, abi.make_camera(f, up, eye), abi.make_uniforms(97, 61, md, emission=2.5), ctx.frame(97, 61))
              for f, up, eye in cams]
+    if classes == 0:
+        # one octree view, so that the batch holds two traversals and the dda
+        # views run in the all-traversal kernel (a batch of one traversal
+        # always runs that traversal's own kernel)
+        svol, smd = env[1]["blocky_svo"]
+        views.append((svol, abi.SVO_NAIVE, abi.make_camera(*cams[4]), abi.make_uniforms(40, 30, smd, emission=2.5),
+                      ctx.frame(40, 30)))
     b = ctx.batch()
     tw = b.tuning()
     b.set_tuning(tw["tile_w"], tw["tiles_per_grab"], waves, classes, tw["interleave"], tw["mix_classes"])

@@ -8,6 +8,8 @@
   request_mix()   served requests: W x H log-uniform in 128..1920 x 128..1080,
                   a uniform traversal among all five, a volume of the kind the
                   traversal needs and an orbit camera at distance U[0.5, 3].
+  path_frame()    which frame of a camera path step k of an n-step run
+                  renders: the steps are spread evenly over the whole path.
 
 Cameras are (forward, up, eye) triples in unit-cube space, the reference's
 camera-file convention (render.txt:33-41).
@@ -17,6 +19,14 @@ import numpy as np
 from xenodon_amd import abi
 
 SEED_MIXED_VIEWS = 0x5EED0004C0FFEE04
+
+
+def path_frame(k, steps, n_frames):
+    """Frame of an n_frames-long camera path rendered by step k of `steps`:
+    round(k * n_frames / steps) (halves up), so any number of steps samples
+    the whole path -- for camera.txt the orbit, the zoom and the frames from
+    inside the volume -- rather than its first frames."""
+    return ((2 * k * n_frames + steps) // (2 * steps)) % n_frames
 
 
 def look_at(eye, target, world_up=(0.0, 1.0, 0.0)):
