@@ -100,9 +100,12 @@ typedef struct {
 /* Message of the last failing call on this thread ("" if none). */
 XD_API const char* xd_last_error(void);
 XD_API int xd_device_count(int* count);
-/* Name and PCI id of a device (xenodon sysinfo, src/sysinfo.cpp:8-80). */
-XD_API int xd_device_describe(int device_index, char* name, size_t name_len, uint32_t* pci_bus, uint32_t* pci_device,
-                              uint64_t* total_bytes);
+/* Name, PCI address (domain:bus:device.function, the fields of
+ * VkPhysicalDevicePCIBusInfoPropertiesEXT that src/sysinfo.cpp:53-59 prints)
+ * and memory size of a device (xenodon sysinfo, src/sysinfo.cpp:8-80).  Any
+ * output pointer may be null. */
+XD_API int xd_device_describe(int device_index, char* name, size_t name_len, uint32_t* pci_domain, uint32_t* pci_bus,
+                              uint32_t* pci_device, uint32_t* pci_function, uint64_t* total_bytes);
 
 /* ---- context (replaces Device/Queue/CommandPool, src/graphics/) ---------- */
 

@@ -214,7 +214,11 @@ def test_cli_argument_errors(tmp_path):
      "Configuration error: Ambiguous key 'vkindex'"),
     ("device {\n vkindex = x\n}\n", "Parse error at line 2, col 13: Expected numeric character, found 'x'"),
     ("device {\n colour = 1\n}\n", "Unexpected key 'colour'"),
-    ("# nothing here\n", "Configuration error: At least one device entry is required"),
+    ("\n \n", "Configuration error: At least one device entry is required"),
+    # headless configs are parsed multiline, where '#' is not a comment
+    # (src/core/Parser.cpp:16-19, src/core/Config.h:140)
+    ("# nothing here\n", "Parse error at line 1, col 1: Unexpected character '#'"),
+    ("device {\n vkindex = 0 # first GPU\n}\n", "Parse error at line 2, col 14: Unexpected character '#'"),
     ("device { vkindex = 0 offset = (0, 0) extent = (4, 4) } trailing", "Unexpected key 'trailing'"),
 ])
 def test_headless_config_errors(tmp_path, text, msg):

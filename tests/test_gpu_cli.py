@@ -17,8 +17,8 @@ from xenodon_amd import io as xio
 pytestmark = pytest.mark.gpu
 
 W, HH = 96, 48
-CONF = """# two regions of one frame on GPU 0
-device {
+# two regions of one frame on GPU 0 (a headless config has no comment syntax)
+CONF = """device {
     vkindex = 0
     offset = (0, 0)
     extent = (40, 48)
@@ -134,3 +134,32 @@ def test_convert_on_the_gpu_matches_oracle(scene, tmp_path, opts, tree, kw):
     nodes, side = xio.load_svo(dst)
     ref, st = xo.build_octree(g, tree=tree, **kw)
     assert side == st["side"] and np.array_equal(nodes, ref)
+
+
+def test_sysinfo_prints_each_device_pci_address():
+    """`xenodon sysinfo` prints every GPU with its PCI address in the format of
+    src/sysinfo.cpp:53-59 ("{:0>8}:{:0>2}:{:0>2}.{}": decimal domain, bus,
+    device and function).  The address is checked against the PCI devices the
+    kernel lists under /sys/bus/pci/devices (hex "dddd:bb:dd.f"), where the
+    box exposes them."""
+    import os
+    import re
+
+    n = abi.device_count()
+    if n < 1:
+        pytest.fail("no HIP device visible: GPU tests need an MI355X")
+    r = subprocess.run([xio.CLI_PATH, "sysinfo"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0] == "System setup information"
+    addrs = re.findall(r"^\tpci bus: (\d{8}):(\d{2,}):(\d{2,})\.(\d+)$", r.stdout, re.M)
+    assert len(addrs) == n, r.stdout
+    assert len(re.findall(r"^GPU \d+:\n\tname: '.+'\n\ttype: DiscreteGpu\n\tpci bus: ", r.stdout, re.M)) == n
+    sysfs = "/sys/bus/pci/devices"
+    listed = set(os.listdir(sysfs)) if os.path.isdir(sysfs) else set()
+    for i, a in enumerate(addrs):
+        dom, bus, dev, fn = (int(v) for v in a)
+        assert (dom, bus, dev, fn) == abi.device_describe(i)[1]
+        assert bus < 256 and dev < 32 and fn < 8
+        if listed:
+            assert f"{dom:04x}:{bus:02x}:{dev:02x}.{fn:x}" in listed, (a, sorted(listed)[:8])

@@ -119,7 +119,8 @@ def _declare(L):
         "xd_last_error": ([], ctypes.c_char_p),
         "xd_device_count": ([P(ctypes.c_int)], ctypes.c_int),
         "xd_device_describe": ([ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, P(ctypes.c_uint32),
-                                P(ctypes.c_uint32), P(ctypes.c_uint64)], ctypes.c_int),
+                                P(ctypes.c_uint32), P(ctypes.c_uint32), P(ctypes.c_uint32), P(ctypes.c_uint64)],
+                               ctypes.c_int),
         "xd_open": ([ctypes.c_int, P(vp)], ctypes.c_int),
         "xd_close": ([vp], ctypes.c_int),
         "xd_context_set_tile": ([vp, ctypes.c_uint32], ctypes.c_int),
@@ -186,6 +187,15 @@ def device_count():
     n = ctypes.c_int()
     st = lib().xd_device_count(ctypes.byref(n))
     return n.value if st == XD_OK else 0
+
+
+def device_describe(index):
+    """(name, (pci domain, bus, device, function), total bytes) of a device."""
+    name = ctypes.create_string_buffer(256)
+    f = [ctypes.c_uint32() for _ in range(4)]
+    total = ctypes.c_uint64()
+    check(lib().xd_device_describe(index, name, 256, *[ctypes.byref(v) for v in f], ctypes.byref(total)))
+    return name.value.decode(), tuple(v.value for v in f), total.value
 
 
 def make_camera(forward, up, translation):

@@ -198,17 +198,16 @@ public:
     Error error(const std::string& what) const {
         return Error(sformat("Parse error at line %zu, col %zu: %s", line_, col_, what.c_str()));
     }
+    // Multiline mode of src/core/Parser.cpp:12-25: a '#' seen by peek() is an
+    // error there (the single-line comment skip is for non-multiline input,
+    // which headless configs never are); consume() reads it as a character.
     int peek() {
-        int c = in_.peek();
-        if (c == '#')
-            while (c != '\n' && c != EOF) {
-                in_.get();
-                c = in_.peek();
-            }
+        const int c = in_.peek();
+        if (c == '#') throw error("Unexpected character '#'");
         return c;
     }
     int consume() {
-        if (peek() == EOF) throw error("Unexpected end of input");
+        if (in_.peek() == EOF) throw error("Unexpected end of input");
         const int c = in_.get();
         if (c == '\n') {
             ++line_;

@@ -94,8 +94,10 @@ struct HeadlessDevice {
 
 // HeadlessConfig from text:
 //   device { vkindex = N  offset = (x, y)  extent = (w, h) }  (repeated)
-// '#' starts a comment to the end of the line.  Errors read
-// "Parse error at line L, col C: ..." or "Configuration error: ...".
+// The file is parsed in the reference's multiline mode, where a '#' is an
+// error ("Unexpected character '#'"), not a comment (src/core/Parser.cpp:12-25,
+// src/core/Config.h:140).  Errors read "Parse error at line L, col C: ..." or
+// "Configuration error: ...".
 std::vector<HeadlessDevice> parse_headless_config(std::istream& in);
 
 }  // namespace xdcli

@@ -171,11 +171,12 @@ void sysinfo() {
     }
     for (int i = 0; i < n; ++i) {
         char name[256] = {};
-        uint32_t bus = 0, dev = 0;
+        uint32_t dom = 0, bus = 0, dev = 0, fn = 0;
         uint64_t bytes = 0;
-        if (xd_device_describe(i, name, sizeof name, &bus, &dev, &bytes) != XD_OK) continue;
+        if (xd_device_describe(i, name, sizeof name, &dom, &bus, &dev, &fn, &bytes) != XD_OK) continue;
         printf("GPU %d:\n\tname: '%s'\n\ttype: DiscreteGpu\n", i, name);
-        printf("\tpci bus: %08u:%02u:%02u.%u\n", 0u, bus, dev, 0u);
+        // "{:0>8}:{:0>2}:{:0>2}.{}" of src/sysinfo.cpp:53-59: decimal fields, zero-padded
+        printf("\tpci bus: %08u:%02u:%02u.%u\n", dom, bus, dev, fn);
         printf("\tNo displays detected\n");
     }
 }

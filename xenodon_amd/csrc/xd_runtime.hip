@@ -181,16 +181,31 @@ int xd_device_count(int* count) {
     return XD_OK;
 }
 
-int xd_device_describe(int device_index, char* name, size_t name_len, uint32_t* pci_bus, uint32_t* pci_device,
-                       uint64_t* total_bytes) {
+int xd_device_describe(int device_index, char* name, size_t name_len, uint32_t* pci_domain, uint32_t* pci_bus,
+                       uint32_t* pci_device, uint32_t* pci_function, uint64_t* total_bytes) {
     hipDeviceProp_t prop;
     hipError_t e = hipGetDeviceProperties(&prop, device_index);
     if (e != hipSuccess) return fail_hip(e, "hipGetDeviceProperties");
     if (name && name_len) {
         snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
     }
-    if (pci_bus) *pci_bus = (uint32_t)prop.pciBusID;
-    if (pci_device) *pci_device = (uint32_t)prop.pciDeviceID;
+    // The properties hold domain, bus and device; the function number is only
+    // in the bus-id string "dddd:bb:dd.f" (hex fields).
+    unsigned dom = (unsigned)prop.pciDomainID, bus = (unsigned)prop.pciBusID, dev = (unsigned)prop.pciDeviceID, fn = 0;
+    char busid[64] = {};
+    if (hipDeviceGetPCIBusId(busid, (int)sizeof busid, device_index) == hipSuccess) {
+        unsigned d2, b2, v2, f2;
+        if (sscanf(busid, "%x:%x:%x.%x", &d2, &b2, &v2, &f2) == 4) {
+            dom = d2;
+            bus = b2;
+            dev = v2;
+            fn = f2;
+        }
+    }
+    if (pci_domain) *pci_domain = dom;
+    if (pci_bus) *pci_bus = bus;
+    if (pci_device) *pci_device = dev;
+    if (pci_function) *pci_function = fn;
     if (total_bytes) *total_bytes = (uint64_t)prop.totalGlobalMem;
     return XD_OK;
 }
