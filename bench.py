@@ -260,18 +260,21 @@ def load_served(srv, volumes):
     return ids
 
 
-def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plain=False, server=None):
+def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plain=False, server=None,
+               max_wait_ms=0.0):
     """Poisson arrivals at `rate` req/s for `duration_s` (or n_saturated
     requests all released at once).  volumes: name -> (kind, data, model_dim).
-    Returns latency percentiles (ms) and aggregate Mrays/s; with plain, also
-    the plain form's rate on the same requests.  server = (server, ids) from
-    load_served reuses resident volumes; otherwise a server is made for the
-    leg."""
+    Returns latency percentiles (ms) and aggregate Mrays/s of the requests
+    served; with plain, also the plain form's rate on the same requests.
+    server = (server, ids) from load_served reuses resident volumes; otherwise
+    a server is made for the leg.  max_wait_ms: the server's latency target
+    for this leg (requests that cannot start within it are shed and counted)."""
     if server is None:
-        srv = ctx.server(tuning=tuning, **SERVER_CONFIG)
+        srv = ctx.server(tuning=tuning, max_wait_ms=max_wait_ms, **SERVER_CONFIG)
         ids = load_served(srv, volumes)
     else:
         srv, ids = server
+        srv.set_max_wait(max_wait_ms)
     st0 = srv.stats()
     n = n_saturated if n_saturated else int(rate * duration_s)
     named, rng = workloads.request_mix({k: v[1] for k, v in ids.items()}, n, seed)
@@ -297,9 +300,16 @@ def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plai
     st = srv.stats()
     if server is None:
         srv.free()
-    bad = [c for c in done if c["status"] != 0]
-    if bad:
-        raise RuntimeError(f"{len(bad)} served requests failed (status {bad[0]['status']})")
+    if server is not None:
+        srv.set_max_wait(0.0)
+    shed = [c for c in done if c["status"] == abi.XD_ERR_REJECTED]
+    bad = [c for c in done if c["status"] not in (abi.XD_OK, abi.XD_ERR_REJECTED)]
+    if bad or (shed and not max_wait_ms):
+        c = (bad or shed)[0]
+        raise RuntimeError(f"{len(bad) + len(shed)} served requests failed (status {c['status']})")
+    done = [c for c in done if c["status"] == abi.XD_OK]
+    if not done:
+        raise RuntimeError(f"all {n} requests were shed under a {max_wait_ms} ms target")
     lat = np.array([c["done_ms"] - c["arrival_ms"] for c in done])
     dev = np.array([c["device_done_ms"] - c["arrival_ms"] for c in done])
     wall = max(c["done_ms"] for c in done) - min(c["arrival_ms"] for c in done)
@@ -309,10 +319,15 @@ def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plai
            "p99_ms": round(float(np.percentile(lat, 99)), 3),
            "p50_device_ms": round(float(np.percentile(dev, 50)), 3),
            "p99_device_ms": round(float(np.percentile(dev, 99)), 3),
-           "mean_rays_per_request": round(rays / n), "launches": st["launches"] - st0["launches"],
+           "mean_rays_per_request": round(rays / len(done)), "launches": st["launches"] - st0["launches"],
            # summed launch durations over the wall time: launches overlap on
            # the device, so this can exceed 1 (it is not a busy fraction)
            "launch_ms_sum_over_wall": round((st["busy_ms"] - st0["busy_ms"]) / wall, 3)}
+    if max_wait_ms:
+        res["max_wait_ms"] = max_wait_ms
+        res["served"] = len(done)
+        res["shed"] = len(shed)
+        res["shed_frac"] = round(len(shed) / n, 4)
     if server is None:
         res["max_requests_per_launch"] = st["max_requests_per_launch"]
     if plain:
@@ -386,6 +401,9 @@ def main():
                     help="also serve requests over the 2048^3 and 1024^3 log-normal volumes next to the others")
     ap.add_argument("--c5-rates", default="50,200,500,1000,2000")
     ap.add_argument("--c5-seconds", type=float, default=10.0)
+    ap.add_argument("--c5-max-wait-ms", type=float, default=100.0,
+                    help="latency target of the 2048^3 served legs run a second time with shedding (0: skip)")
+    ap.add_argument("--c5-target-rates", default="1000,2000")
     ap.add_argument("--serve-saturated", type=int, default=400,
                     help="requests released at once for the served form's peak rate (0: skip)")
     args = ap.parse_args()
@@ -717,6 +735,16 @@ def main():
             res = served_leg(ctx, c5vols, r, args.c5_seconds, seed=2000 + k, tuning=serve_tuning, server=(srv, ids))
             log(f"served (2048^3 resident) @ {r:.0f} req/s: {res['mrays_s']} Mrays/s, p99 {res['p99_ms']} ms")
             served_c5.append(res)
+        if args.c5_max_wait_ms > 0:
+            # the same arrivals with a latency target: what cannot start within
+            # it is shed instead of queued
+            for k, r in enumerate(float(x) for x in args.c5_target_rates.split(",") if x):
+                res = served_leg(ctx, c5vols, r, args.c5_seconds, seed=2100 + k, tuning=serve_tuning,
+                                 server=(srv, ids), max_wait_ms=args.c5_max_wait_ms)
+                log(f"served (2048^3 resident, target {args.c5_max_wait_ms:.0f} ms) @ {r:.0f} req/s: "
+                    f"{res['mrays_s']} Mrays/s, p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms, "
+                    f"shed {res['shed_frac']:.1%}")
+                served_c5.append(res)
         res = served_leg(ctx, c5vols, 0, 0, seed=2999, tuning=serve_tuning, n_saturated=args.serve_saturated or 400,
                          plain=True, server=(srv, ids))
         log(f"served (2048^3 resident), requests at once: {res['mrays_s']} Mrays/s; plain "

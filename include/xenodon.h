@@ -219,7 +219,11 @@ typedef struct {
     uint32_t class_kernels;
     uint32_t interleave;
     uint32_t mix_classes; /* all-traversal kernel: 1 = its waves start on different
-                             traversals, so each CU runs a mix of them at once */
+                             traversals, so each CU runs a mix of them at once;
+                             2 = as 1, and the XCDs are shared out between the
+                             traversals in proportion to their work, so that
+                             each XCD's L2 serves one or two traversals' data
+                             (batches only; the served form reads 2 as 1) */
 } xd_batch_tuning;
 
 typedef struct xd_batch xd_batch;
@@ -270,6 +274,14 @@ typedef struct {
     xd_batch_tuning tuning;      /* tile shapes etc.; zero-filled = defaults.  A
                                     step whose tiles are all of one traversal
                                     runs that traversal's own kernel */
+    double max_wait_ms;          /* latency target; 0 = none (every request waits
+                                    for its turn).  With a target, a request is
+                                    admitted only while the rays the admitted
+                                    requests still have to render take at most
+                                    max_wait_ms at the device's measured rate,
+                                    and a request that has waited max_wait_ms
+                                    for admission is shed: it completes with
+                                    status XD_ERR_REJECTED and no frame */
 } xd_server_config;
 
 typedef struct {
@@ -303,6 +315,8 @@ typedef struct {
     uint64_t completed, launches, tiles, rays;
     uint32_t max_requests_per_launch, max_launches_overlapped;
     double busy_ms;              /* sum of launch durations (HIP events) */
+    uint64_t shed;               /* requests shed under max_wait_ms */
+    double rays_per_ms;          /* measured device rate the admission uses */
 } xd_server_stats;
 
 XD_API int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** out);
@@ -322,6 +336,9 @@ XD_API int xd_server_submit(xd_server* srv, const xd_request* req, uint64_t* req
  * (0 = do not wait, <0 = wait for at least one). */
 XD_API int xd_server_poll(xd_server* srv, xd_completion* out, uint32_t max, uint32_t* n, int32_t timeout_ms);
 XD_API int xd_server_stats_get(xd_server* srv, xd_server_stats* out);
+/* Changes the latency target (xd_server_config.max_wait_ms) of a running
+ * server; it applies from the next admission on. */
+XD_API int xd_server_set_max_wait(xd_server* srv, double max_wait_ms);
 /* Current time on the server clock (ms). */
 XD_API double xd_server_now_ms(const xd_server* srv);
 /* Finishes every queued request, stops the threads, frees the volumes. */
@@ -348,12 +365,12 @@ typedef struct {
  * malloc'd into *out_nodes and released with xd_free. */
 XD_API int xd_build_octree(const void* rgba8, uint64_t dim_x, uint64_t dim_y, uint64_t dim_z, int heuristic,
                            int chan_diff, double std_dev, int tree_type, uint32_t** out_nodes, xd_build_stats* stats);
-/* The same build on the GPU from a grid volume already in HBM, for the
- * channel-difference heuristic and sparse or rope trees (DAGs and the std-dev
- * heuristic: xd_build_octree).  The node buffer and stats are identical to
+/* The same build on the GPU from a grid volume already in HBM (`xenodon
+ * convert --gpu`), for both heuristics and sparse, DAG and rope trees
+ * (src/convert.cpp:13-122).  The node buffer and stats are identical to
  * xd_build_octree's; the buffer is malloc'd into *out_nodes (xd_free). */
-XD_API int xd_build_octree_device(xd_context* ctx, const xd_volume* grid, int chan_diff, int tree_type,
-                                  uint32_t** out_nodes, xd_build_stats* stats);
+XD_API int xd_build_octree_device(xd_context* ctx, const xd_volume* grid, int heuristic, int chan_diff, double std_dev,
+                                  int tree_type, uint32_t** out_nodes, xd_build_stats* stats);
 XD_API void xd_free(void* p);
 
 #ifdef __cplusplus

@@ -198,8 +198,7 @@ def test_cli_argument_errors(tmp_path):
          "Error: --std-dev and --chan-diff are mutually exclusive"),
         (["convert", "missing.tif", "o.svo"], "Error reading 'missing.tif': Failed to open"),
         (["convert", "--gpu", "x", "a", "b"], "Error: Invalid value for <gpu index> of parameter --gpu"),
-        (["convert", "--gpu", "0", "--dag", "a", "b"],
-         "Error: --gpu builds sparse and rope trees with the channel-difference heuristic only"),
+        (["convert", "--gpu", "0", "--dag", "--rope", "a", "b"], "Error: --dag and --rope are mutually exclusive"),
         (["help", "a", "b"], "Error: Invalid usage of subcommand 'help'"),
         (["frobnicate"], "Error: Invalid subcommand 'frobnicate'"),
     ]

@@ -120,7 +120,10 @@ def test_discard_output_and_errors(scene):
 
 
 @pytest.mark.parametrize("opts,tree,kw", [([], xo.TREE_SPARSE, {}),
-                                          (["--rope", "--chan-diff", "20"], xo.TREE_ROPE, {"chan_diff": 20})])
+                                          (["--rope", "--chan-diff", "20"], xo.TREE_ROPE, {"chan_diff": 20}),
+                                          (["--dag"], xo.TREE_DAG, {}),
+                                          (["--dag", "--std-dev", "30.5"], xo.TREE_DAG,
+                                           {"heuristic": xo.HEUR_STD_DEV, "std_dev": 30.5})])
 def test_convert_on_the_gpu_matches_oracle(scene, tmp_path, opts, tree, kw):
     """`xenodon convert --gpu 0` builds on the device and writes the same
     .svo as the reference converter's restatement."""

@@ -82,7 +82,7 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("frame_arena_bytes", ctypes.c_uint64),
                 ("max_active", ctypes.c_uint32), ("tiles_per_launch", ctypes.c_uint32),
                 ("launches_in_flight", ctypes.c_uint32), ("readback_threads", ctypes.c_uint32),
-                ("tuning", BatchTuning)]
+                ("tuning", BatchTuning), ("max_wait_ms", ctypes.c_double)]
 
 
 class Request(ctypes.Structure):
@@ -103,7 +103,8 @@ class ServerStats(ctypes.Structure):
                 ("volumes", ctypes.c_uint32), ("pending", ctypes.c_uint32), ("active", ctypes.c_uint32),
                 ("reserved", ctypes.c_uint32), ("completed", ctypes.c_uint64), ("launches", ctypes.c_uint64),
                 ("tiles", ctypes.c_uint64), ("rays", ctypes.c_uint64), ("max_requests_per_launch", ctypes.c_uint32),
-                ("max_launches_overlapped", ctypes.c_uint32), ("busy_ms", ctypes.c_double)]
+                ("max_launches_overlapped", ctypes.c_uint32), ("busy_ms", ctypes.c_double),
+                ("shed", ctypes.c_uint64), ("rays_per_ms", ctypes.c_double)]
 
 
 HEURISTIC_CHAN_DIFF, HEURISTIC_STD_DEV = 0, 1
@@ -138,7 +139,8 @@ def _declare(L):
         "xd_build_octree": ([vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
                              ctypes.c_double, ctypes.c_int, P(vp), P(BuildStats)], ctypes.c_int),
         "xd_free": ([vp], None),
-        "xd_build_octree_device": ([vp, vp, ctypes.c_int, ctypes.c_int, P(vp), P(BuildStats)], ctypes.c_int),
+        "xd_build_octree_device": ([vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, P(vp),
+                                    P(BuildStats)], ctypes.c_int),
         "xd_batch_create": ([vp, P(vp)], ctypes.c_int),
         "xd_batch_set_views": ([vp, P(View), ctypes.c_uint32], ctypes.c_int),
         "xd_batch_set_tuning": ([vp, P(BatchTuning)], ctypes.c_int),
@@ -149,6 +151,7 @@ def _declare(L):
         "xd_volume_footprint": ([vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                  ctypes.c_uint64, P(ctypes.c_uint64)], ctypes.c_int),
         "xd_server_create": ([vp, P(ServerConfig), P(vp)], ctypes.c_int),
+        "xd_server_set_max_wait": ([vp, ctypes.c_double], ctypes.c_int),
         "xd_server_load_grid": ([vp, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_uint32)],
                                 ctypes.c_int),
         "xd_server_load_svo": ([vp, vp, ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_uint32)], ctypes.c_int),
@@ -275,8 +278,9 @@ class Context:
         return self._adopt(Batch(self, h))
 
     def server(self, hbm_budget_bytes=0, frame_arena_bytes=0, max_active=0, tiles_per_launch=0,
-               launches_in_flight=0, readback_threads=0, tuning=None):
+               launches_in_flight=0, readback_threads=0, tuning=None, max_wait_ms=0.0):
         c = ServerConfig()
+        c.max_wait_ms = float(max_wait_ms)
         c.hbm_budget_bytes = hbm_budget_bytes
         c.frame_arena_bytes = frame_arena_bytes
         c.max_active = max_active
@@ -294,12 +298,13 @@ class Context:
         check(lib().xd_server_create(self.h, ctypes.byref(c), ctypes.byref(h)))
         return self._adopt(Server(self, h))
 
-    def build_octree(self, grid_volume, chan_diff=0, tree=None):
-        """GPU octree build from a grid volume in HBM (sparse or rope trees,
-        channel-difference heuristic); same (nodes, stats) as build_octree."""
+    def build_octree(self, grid_volume, chan_diff=0, tree=None, heuristic=None, std_dev=0.0):
+        """GPU octree build from a grid volume in HBM (both heuristics; sparse,
+        DAG or rope trees); same (nodes, stats) as build_octree."""
         out = ctypes.c_void_p()
         st = BuildStats()
-        check(lib().xd_build_octree_device(self.h, grid_volume.h, chan_diff, TREE_SPARSE if tree is None else tree,
+        check(lib().xd_build_octree_device(self.h, grid_volume.h, HEURISTIC_CHAN_DIFF if heuristic is None else heuristic,
+                                           chan_diff, float(std_dev), TREE_SPARSE if tree is None else tree,
                                            ctypes.byref(out), ctypes.byref(st)))
         return _take_nodes(out, st)
 
@@ -337,6 +342,29 @@ class Context:
             f.free()
 
 
+def view_array(views):
+    """ctypes xd_view array of (volume, traversal, camera, uniforms, frame) tuples."""
+    arr = (View * max(1, len(views)))()
+    for i, (vol, t, cam, uni, fr) in enumerate(views):
+        arr[i].volume = vol.h
+        arr[i].traversal = t
+        arr[i].camera = cam
+        arr[i].uniforms = uni
+        arr[i].frame = fr.h
+    return arr
+
+
+def batch_tuning(tile_w, tiles_per_grab=1, waves_per_simd=8, class_kernels=0, interleave=2, mix_classes=1):
+    t = BatchTuning()
+    t.tile_w[:] = list(tile_w)
+    t.tiles_per_grab = tiles_per_grab
+    t.waves_per_simd = waves_per_simd
+    t.class_kernels = class_kernels
+    t.interleave = interleave
+    t.mix_classes = mix_classes
+    return t
+
+
 class Batch:
     """Persistent ray-tile-queue launch over many views (xd_batch_*)."""
 
@@ -347,13 +375,7 @@ class Batch:
 
     def set_views(self, views):
         """views: list of (volume, traversal, camera, uniforms, frame)."""
-        arr = (View * max(1, len(views)))()
-        for i, (vol, t, cam, uni, fr) in enumerate(views):
-            arr[i].volume = vol.h
-            arr[i].traversal = t
-            arr[i].camera = cam
-            arr[i].uniforms = uni
-            arr[i].frame = fr.h
+        arr = view_array(views)
         self._keep = list(views)
         check(lib().xd_batch_set_views(self.h, arr, len(views)))
 
@@ -365,13 +387,7 @@ class Batch:
 
     def set_tuning(self, tile_w, tiles_per_grab=1, waves_per_simd=8, class_kernels=0, interleave=2, mix_classes=1):
         """Defaults are the library's starting launch shape."""
-        t = BatchTuning()
-        t.tile_w[:] = list(tile_w)
-        t.tiles_per_grab = tiles_per_grab
-        t.waves_per_simd = waves_per_simd
-        t.class_kernels = class_kernels
-        t.interleave = interleave
-        t.mix_classes = mix_classes
+        t = batch_tuning(tile_w, tiles_per_grab, waves_per_simd, class_kernels, interleave, mix_classes)
         check(lib().xd_batch_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, trials=3):
@@ -457,6 +473,10 @@ class Server:
                         "arrival_ms": c.arrival_ms, "start_ms": c.start_ms, "device_done_ms": c.device_done_ms,
                         "done_ms": c.done_ms, "rays": c.rays, "image": self._dst.pop(c.request_id, None)})
         return out
+
+    def set_max_wait(self, ms):
+        """Latency target (ms) for admissions from now on; 0 = none."""
+        check(lib().xd_server_set_max_wait(self.h, float(ms)))
 
     def stats(self):
         st = ServerStats()

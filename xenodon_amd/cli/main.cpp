@@ -214,10 +214,6 @@ void convert(const std::vector<std::string>& args) {
         printf("Error: --std-dev and --chan-diff are mutually exclusive\n");
         return;
     }
-    if (gpu >= 0 && (dag || stddev >= 0)) {
-        printf("Error: --gpu builds sparse and rope trees with the channel-difference heuristic only\n");
-        return;
-    }
     printf("Loading source...\n");
     fflush(stdout);
     GridData grid;
@@ -244,7 +240,8 @@ void convert(const std::vector<std::string>& args) {
         int rc = xd_open((int)gpu, &ctx);
         if (rc == XD_OK) rc = xd_volume_from_grid(ctx, grid.texels.data(), grid.dim[0], grid.dim[1], grid.dim[2], &vol);
         if (rc == XD_OK)
-            rc = xd_build_octree_device(ctx, vol, (int)std::max<int64_t>(chan_diff, 0), type, &nodes, &st);
+            rc = xd_build_octree_device(ctx, vol, heuristic, (int)std::max<int64_t>(chan_diff, 0), std::max(stddev, 0.0),
+                                        type, &nodes, &st);
         const std::string msg = rc == XD_OK ? "" : xd_last_error();
         xd_close(ctx);
         if (rc != XD_OK) {

@@ -246,13 +246,67 @@ int ensure_capacity(xd_batch* b, uint32_t nviews, uint32_t ntiles) {
     return XD_OK;
 }
 
-// Builds and uploads the view and tile tables for the current views/tuning;
-// *need gets each class's LDS stack stride and, in grid[c], 1 if class c has
-// tiles.  Tiles are grouped by class, and within a class dealt into kQueues
-// ranges by the band of their view they lie in.
-int upload_tables(xd_batch* b, BatchClasses* need) {
-    const uint32_t n = (uint32_t)b->views.size();
-    std::vector<BatchView> bv(n);
+// Relative cost of a ray per traversal, from the one-traversal batched rates
+// on the 512x361x512 CT volume at 1920x1080 (dda 1604, svo-naive 1490, esvo
+// 1227, svo-df 740, svo-rope 1514 Mrays/s).  Only the split of the XCDs
+// between traversals depends on it; the tiles a wrong split leaves over are
+// taken by the waves of the other XCDs.
+constexpr double kRayCost[kClasses] = {1.0, 1.08, 1.31, 2.17, 1.06};
+
+// XCD affinity: the eight XCDs are shared out between the traversal classes
+// in proportion to each class's tiles x ray cost, as consecutive segments of
+// one line of 8 XCD-widths, so that each XCD's L2 serves one or two
+// traversals' data.  A class's tiles (its band ranges concatenated, so still
+// in image order) are cut into runs, one per XCD its segment covers, in
+// proportion to the cover.  The queue walk is unchanged: a wave drains its own
+// XCD's ranges (of every class) first, then the next XCD's.
+void deal_by_affinity(std::vector<std::vector<uint32_t>>& q) {
+    double w[kClasses], total = 0;
+    for (uint32_t c = 0; c < kClasses; ++c) {
+        size_t n = 0;
+        for (uint32_t b = 0; b < kQueues; ++b) n += q[c * kQueues + b].size();
+        w[c] = (double)n * kRayCost[c];
+        total += w[c];
+    }
+    if (total <= 0) return;
+    double at = 0;  // start of class c's segment, in XCD widths
+    for (uint32_t c = 0; c < kClasses; ++c) {
+        std::vector<uint32_t> all;
+        for (uint32_t b = 0; b < kQueues; ++b) {
+            auto& v = q[c * kQueues + b];
+            all.insert(all.end(), v.begin(), v.end());
+            v.clear();
+        }
+        const double len = w[c] / total * kQueues;
+        const double end = at + len;
+        size_t taken = 0;
+        for (uint32_t x = (uint32_t)at; x < kQueues && x < end && !all.empty(); ++x) {
+            const double cover = std::min(end, (double)x + 1) - std::max(at, (double)x);
+            const bool last = (double)x + 1 >= end - 1e-9 || x + 1 == kQueues;
+            const size_t k = last ? all.size() - taken
+                                  : std::min(all.size() - taken, (size_t)std::llround(cover / len * (double)all.size()));
+            q[c * kQueues + x].assign(all.begin() + taken, all.begin() + taken + k);
+            taken += k;
+        }
+        at = end;
+    }
+}
+
+// The view and tile tables of a launch, built on the host.
+struct HostTables {
+    std::vector<BatchView> views;
+    std::vector<uint32_t> tiles;
+    uint32_t qbeg[kClasses * kQueues + 1];
+    BatchClasses cls;  // grid[c] = 1 if kernel c has tiles; slots[c] its LDS stack stride
+    uint64_t total_rays;
+};
+
+// Builds the view and tile tables of n views under a tuning.  Tiles are
+// grouped by class, and within a class dealt into kQueues ranges by the band
+// of their view they lie in.
+int build_tables(const xd_view* views, uint32_t n, const xd_batch_tuning& tuning, HostTables& T) {
+    T.views.assign(n, BatchView{});
+    std::vector<BatchView>& bv = T.views;
     std::vector<std::vector<uint32_t>> q(kClasses * kQueues);
     struct Group {
         uint64_t key[6];
@@ -261,13 +315,13 @@ int upload_tables(xd_batch* b, BatchClasses* need) {
     };
     std::vector<Group> groups;
     BatchClasses cls{};
-    b->total_rays = 0;
+    T.total_rays = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        const xd_view& v = b->views[i];
+        const xd_view& v = views[i];
         BatchView& B = bv[i];
         uint64_t nt = 0;
         make_batch_view(v.volume, v.traversal, &v.camera, &v.uniforms, v.frame->d_pixels, v.frame->w,
-                        b->tuning.tile_w[v.traversal], B, &nt);
+                        tuning.tile_w[v.traversal], B, &nt);
         const uint32_t W = v.uniforms.output_region.extent[0], H = v.uniforms.output_region.extent[1];
         const uint32_t tiles_y = (uint32_t)(nt / B.tiles_x);
         if (nt > kMaxTilesPerView)
@@ -276,7 +330,7 @@ int upload_tables(xd_batch* b, BatchClasses* need) {
         const uint32_t c = (uint32_t)v.traversal;
         cls.slots[c] = std::max(cls.slots[c], batch_view_slots(B));
         cls.grid[c] = 1;
-        const uint32_t order = b->tuning.interleave;
+        const uint32_t order = tuning.interleave;
         if (order == 1 || order == 5) {
             // views of one volume, traversal, size and tile shape form a group
             // whose tiles are dealt together below
@@ -298,11 +352,11 @@ int upload_tables(xd_batch* b, BatchClasses* need) {
                             q[c * kQueues + band].push_back((i << 20) | (ty * B.tiles_x + tx));
             }
         }
-        b->total_rays += (uint64_t)W * H;
+        T.total_rays += (uint64_t)W * H;
     }
     // interleaved: the same tile of every view of a group, one after another
     // (order 5: in squares of 8 x 8 tiles)
-    const uint32_t gsq = b->tuning.interleave == 5 ? 8u : 1u;
+    const uint32_t gsq = tuning.interleave == 5 ? 8u : 1u;
     for (const auto& G : groups)
         for (uint32_t ty0 = 0; ty0 < G.tiles_y; ty0 += gsq) {
             const uint32_t band = (uint32_t)((uint64_t)ty0 * kQueues / G.tiles_y);
@@ -318,7 +372,7 @@ int upload_tables(xd_batch* b, BatchClasses* need) {
     // that loads the next steps' macro words during the current gathers).
     uint32_t present = 0;
     for (uint32_t c = 0; c < kClasses; ++c) present += cls.grid[c] ? 1u : 0u;
-    const bool per_class = b->tuning.class_kernels || present == 1;
+    const bool per_class = tuning.class_kernels || present == 1;
     // one LDS layout per kernel: every wave's stack region is sized for the
     // deepest tree among the views that kernel drains
     if (!per_class) {
@@ -333,24 +387,38 @@ int upload_tables(xd_batch* b, BatchClasses* need) {
     }
     for (uint32_t i = 0; i < n; ++i)
         bv[i].p.lds_slots = std::max<uint32_t>(cls.slots[per_class ? bv[i].traversal : kClasses], 1);
-    uint32_t qbeg[kClasses * kQueues + 1];
-    uint32_t total = 0;
+    if (!per_class && tuning.mix_classes == 2) deal_by_affinity(q);
+    uint64_t total = 0;
     for (uint32_t k = 0; k < kClasses * kQueues; ++k) {
-        qbeg[k] = total;
-        total += (uint32_t)q[k].size();
+        T.qbeg[k] = (uint32_t)total;
+        total += q[k].size();
     }
-    qbeg[kClasses * kQueues] = total;
-    int rc = ensure_capacity(b, n, std::max<uint32_t>(total, 1));
+    if (total > 0xFFFFFFFFull) return fail(XD_ERR_INVALID_ARGUMENT, "%llu wave tiles in one batch", (unsigned long long)total);
+    T.qbeg[kClasses * kQueues] = (uint32_t)total;
+    T.tiles.clear();
+    T.tiles.reserve(total);
+    for (auto& v : q) T.tiles.insert(T.tiles.end(), v.begin(), v.end());
+    T.cls = cls;
+    return XD_OK;
+}
+
+// Builds the current views' tables and uploads them into the batch's own
+// device buffers; *need gets each kernel's LDS stack stride and, in grid[c],
+// 1 if kernel c has tiles.
+int upload_tables(xd_batch* b, BatchClasses* need) {
+    HostTables T;
+    const uint32_t n = (uint32_t)b->views.size();
+    int rc = build_tables(b->views.data(), n, b->tuning, T);
     if (rc) return rc;
-    std::vector<uint32_t> flat;
-    flat.reserve(total);
-    for (auto& v : q) flat.insert(flat.end(), v.begin(), v.end());
+    b->total_rays = T.total_rays;
+    const uint32_t total = (uint32_t)T.tiles.size();
+    if ((rc = ensure_capacity(b, n, std::max<uint32_t>(total, 1)))) return rc;
     hipError_t e;
-    if ((e = hipMemcpy(b->d_views, bv.data(), sizeof(BatchView) * n, hipMemcpyHostToDevice)) != hipSuccess ||
-        (total && (e = hipMemcpy(b->d_tiles, flat.data(), sizeof(uint32_t) * total, hipMemcpyHostToDevice)) != hipSuccess) ||
-        (e = hipMemcpy(b->d_qbeg, qbeg, sizeof(qbeg), hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = hipMemcpy(b->d_views, T.views.data(), sizeof(BatchView) * n, hipMemcpyHostToDevice)) != hipSuccess ||
+        (total && (e = hipMemcpy(b->d_tiles, T.tiles.data(), sizeof(uint32_t) * total, hipMemcpyHostToDevice)) != hipSuccess) ||
+        (e = hipMemcpy(b->d_qbeg, T.qbeg, sizeof(T.qbeg), hipMemcpyHostToDevice)) != hipSuccess)
         return fail_hip(e, "batch tables");
-    *need = cls;
+    *need = T.cls;
     return XD_OK;
 }
 
@@ -408,6 +476,51 @@ int run(xd_batch* b, double* ms) {
     return XD_OK;
 }
 
+// Every view's volume and frame belong to ctx, its traversal suits its
+// volume, and its frame is the size of its output region.
+int check_views(const xd_context* ctx, const xd_view* views, uint32_t count) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const xd_view& v = views[i];
+        if (!v.volume || !v.frame) return fail(XD_ERR_INVALID_ARGUMENT, "view %u: null volume or frame", i);
+        if (v.volume->ctx != ctx || v.frame->ctx != ctx)
+            return fail(XD_ERR_INVALID_ARGUMENT, "view %u: volume or frame belongs to another context", i);
+        int rc = check_traversal(v.traversal, v.volume);
+        if (rc) return rc;
+        if ((rc = check_view(&v.uniforms))) return rc;
+        if (v.frame->w != v.uniforms.output_region.extent[0] || v.frame->h != v.uniforms.output_region.extent[1])
+            return fail(XD_ERR_INVALID_ARGUMENT, "view %u: frame is %ux%u but the output region is %ux%u", i,
+                        v.frame->w, v.frame->h, v.uniforms.output_region.extent[0], v.uniforms.output_region.extent[1]);
+    }
+    return XD_OK;
+}
+
+// The fastest launch shape measured on the 1080p five-traversal batch: one
+// kernel for every traversal, its waves starting on different traversals,
+// 8 waves per SIMD, tiles in squares of 8.
+xd_batch_tuning default_tuning() {
+    xd_batch_tuning t{};
+    for (int i = 0; i < 5; ++i) t.tile_w[i] = 8;
+    t.tiles_per_grab = 1;
+    t.interleave = 2;
+    t.waves_per_simd = 8;
+    t.mix_classes = 1;
+    return t;
+}
+
+int check_tuning(const xd_batch_tuning* t) {
+    for (int i = 0; i < 5; ++i)
+        if (t->tile_w[i] != 8 && t->tile_w[i] != 16 && t->tile_w[i] != 32)
+            return fail(XD_ERR_INVALID_ARGUMENT, "tile width %u is not 8, 16 or 32", t->tile_w[i]);
+    if (t->tiles_per_grab < 1 || t->tiles_per_grab > 8)
+        return fail(XD_ERR_INVALID_ARGUMENT, "tiles_per_grab must be 1..8");
+    if (t->waves_per_simd != 0 && t->waves_per_simd != 8)
+        return fail(XD_ERR_INVALID_ARGUMENT, "waves_per_simd must be 0 (compiler's choice) or 8");
+    if (t->class_kernels > 1) return fail(XD_ERR_INVALID_ARGUMENT, "class_kernels must be 0 or 1");
+    if (t->interleave > 5) return fail(XD_ERR_INVALID_ARGUMENT, "interleave (tile order) must be 0..5");
+    if (t->mix_classes > 2) return fail(XD_ERR_INVALID_ARGUMENT, "mix_classes must be 0, 1 or 2");
+    return XD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -419,14 +532,7 @@ int xd_batch_create(xd_context* ctx, xd_batch** out) {
     if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
     auto* b = new xd_batch();
     b->ctx = ctx;
-    // the fastest launch shape measured on the 1080p five-traversal batch:
-    // one kernel for every traversal, its waves starting on different
-    // traversals, 8 waves per SIMD, tiles in squares of 8
-    for (int t = 0; t < 5; ++t) b->tuning.tile_w[t] = 8;
-    b->tuning.tiles_per_grab = 1;
-    b->tuning.interleave = 2;
-    b->tuning.waves_per_simd = 8;
-    b->tuning.mix_classes = 1;
+    b->tuning = default_tuning();
     if ((e = hipMalloc(&b->d_qbeg, sizeof(uint32_t) * (kClasses * kQueues + 1))) != hipSuccess ||
         (e = hipMalloc(&b->d_counters, sizeof(uint32_t) * kClasses * kQueues)) != hipSuccess) {
         xd_batch_free(b);
@@ -440,18 +546,8 @@ int xd_batch_create(xd_context* ctx, xd_batch** out) {
 int xd_batch_set_views(xd_batch* b, const xd_view* views, uint32_t count) {
     if (!b || (!views && count)) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
     if (count > kMaxBatchViews) return fail(XD_ERR_INVALID_ARGUMENT, "%u views exceed the limit of %u", count, kMaxBatchViews);
-    for (uint32_t i = 0; i < count; ++i) {
-        const xd_view& v = views[i];
-        if (!v.volume || !v.frame) return fail(XD_ERR_INVALID_ARGUMENT, "view %u: null volume or frame", i);
-        if (v.volume->ctx != b->ctx || v.frame->ctx != b->ctx)
-            return fail(XD_ERR_INVALID_ARGUMENT, "view %u: volume or frame belongs to another context", i);
-        int rc = check_traversal(v.traversal, v.volume);
-        if (rc) return rc;
-        if ((rc = check_view(&v.uniforms))) return rc;
-        if (v.frame->w != v.uniforms.output_region.extent[0] || v.frame->h != v.uniforms.output_region.extent[1])
-            return fail(XD_ERR_INVALID_ARGUMENT, "view %u: frame is %ux%u but the output region is %ux%u", i,
-                        v.frame->w, v.frame->h, v.uniforms.output_region.extent[0], v.uniforms.output_region.extent[1]);
-    }
+    int rc = check_views(b->ctx, views, count);
+    if (rc) return rc;
     hipError_t e = hipSetDevice(b->ctx->device);
     if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
     b->views.assign(views, views + count);
@@ -460,16 +556,8 @@ int xd_batch_set_views(xd_batch* b, const xd_view* views, uint32_t count) {
 
 int xd_batch_set_tuning(xd_batch* b, const xd_batch_tuning* t) {
     if (!b || !t) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
-    for (int i = 0; i < 5; ++i)
-        if (t->tile_w[i] != 8 && t->tile_w[i] != 16 && t->tile_w[i] != 32)
-            return fail(XD_ERR_INVALID_ARGUMENT, "tile width %u is not 8, 16 or 32", t->tile_w[i]);
-    if (t->tiles_per_grab < 1 || t->tiles_per_grab > 8)
-        return fail(XD_ERR_INVALID_ARGUMENT, "tiles_per_grab must be 1..8");
-    if (t->waves_per_simd != 0 && t->waves_per_simd != 8)
-        return fail(XD_ERR_INVALID_ARGUMENT, "waves_per_simd must be 0 (compiler's choice) or 8");
-    if (t->class_kernels > 1) return fail(XD_ERR_INVALID_ARGUMENT, "class_kernels must be 0 or 1");
-    if (t->interleave > 5) return fail(XD_ERR_INVALID_ARGUMENT, "interleave (tile order) must be 0..5");
-    if (t->mix_classes > 1) return fail(XD_ERR_INVALID_ARGUMENT, "mix_classes must be 0 or 1");
+    int rc = check_tuning(t);
+    if (rc) return rc;
     b->tuning = *t;
     if (b->views.empty()) return XD_OK;
     hipError_t e = hipSetDevice(b->ctx->device);
@@ -554,7 +642,7 @@ int xd_batch_autotune(xd_batch* b, uint32_t trials, xd_batch_tuning* chosen) {
     // then the launch shape, on the whole batch, one field at a time from
     // the starting tuning
     b->views = all;
-    if (!rc) rc = sweep(b, trials, &b->tuning.mix_classes, {0u, 1u});
+    if (!rc) rc = sweep(b, trials, &b->tuning.mix_classes, {0u, 1u, 2u});
     if (!rc) rc = sweep(b, trials, &b->tuning.interleave, {0u, 1u, 2u, 3u, 4u, 5u});
     if (!rc) rc = sweep(b, trials, &b->tuning.waves_per_simd, {0u, 8u});
     if (!rc) rc = sweep(b, trials, &b->tuning.class_kernels, {0u, 1u});

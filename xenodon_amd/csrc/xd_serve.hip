@@ -156,6 +156,11 @@ struct xd_server {
     bool class_kernels = false;
     bool mix = false;
 
+    // admission under a latency target: rays still to render by admitted
+    // requests, and the device rate measured over finished steps
+    uint64_t outstanding_rays = 0;
+    double rays_per_ms = 0;
+
     // state
     bool stop = false, reader_stop = false;
     std::string broken;  // first device error; the server then refuses work
@@ -189,18 +194,41 @@ void abort_all(xd_server* s, int status) {
         }
     }
     s->active.swap(keep);
+    s->outstanding_rays = 0;  // what is kept has all its tiles done
     s->cv_done.notify_all();
 }
 
+// Admits released requests oldest first while slots and frame memory last.
+// Under a latency target (max_wait_ms > 0) a request is also held back while
+// the admitted requests' remaining rays would take longer than the target at
+// the measured rate, and a request that has waited longer than the target is
+// shed.  Until a first step has been measured the rate is unknown, and one
+// request at a time is admitted.
 void admit(xd_server* s, double now) {
+    const double target = s->cfg.max_wait_ms;
     for (auto it = s->pending.begin(); it != s->pending.end();) {
         Request* q = it->second;
         if (q->c.arrival_ms > now) break;  // not released yet (and neither is any later one)
+        if (target > 0 && now - q->c.arrival_ms > target) {
+            q->c.status = XD_ERR_REJECTED;
+            q->c.done_ms = now;
+            s->vols[q->vol_id].refs--;
+            s->st.shed++;
+            s->done.push_back(q->c);
+            delete q;
+            it = s->pending.erase(it);
+            s->cv_done.notify_all();
+            continue;
+        }
         if (s->active.size() >= s->cfg.max_active) break;
+        if (target > 0 && !s->active.empty() &&
+            (s->rays_per_ms <= 0 || (double)(s->outstanding_rays + q->c.rays) > s->rays_per_ms * target))
+            break;
         uint64_t off = 0;
         if (!s->arena.alloc(q->frame_bytes, &off)) break;  // oldest first: wait for memory
         q->arena_off = off;
         q->bv.p.out = reinterpret_cast<uint32_t*>(s->arena_dev + off);
+        s->outstanding_rays += q->c.rays;
         s->active.push_back(q);
         it = s->pending.erase(it);
     }
@@ -316,15 +344,29 @@ bool launch_step(xd_server* s, Step& S, std::string* err) {
 
 void finish_step(xd_server* s, Step& S) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, S.ev0, S.ev1) == hipSuccess) s->st.busy_ms += ms;
+    const bool timed = hipEventElapsedTime(&ms, S.ev0, S.ev1) == hipSuccess;
+    if (timed) s->st.busy_ms += ms;
     const double now = ms_since(s->t0);
+    uint64_t step_rays = 0;
     for (const Part& p : S.parts) {
         Request* q = p.req;
+        // the part's share of the request's rays, by tiles
+        const uint64_t r = q->tiles_total ? q->c.rays * (q->tiles_done + p.count) / q->tiles_total -
+                                                q->c.rays * q->tiles_done / q->tiles_total
+                                          : 0;
+        step_rays += r;
+        s->outstanding_rays -= std::min(s->outstanding_rays, r);
         q->tiles_done += p.count;
         if (q->tiles_done == q->tiles_total) {
             q->c.device_done_ms = now;
             s->readq.push_back(q);
         }
+    }
+    // the device rate (rays per ms of launch time), smoothed over steps
+    if (timed && ms > 0) {
+        const double rate = (double)step_rays / ms;
+        s->rays_per_ms = s->rays_per_ms > 0 ? 0.8 * s->rays_per_ms + 0.2 * rate : rate;
+        s->st.rays_per_ms = s->rays_per_ms;
     }
     S.parts.clear();
     S.busy = false;
@@ -495,6 +537,7 @@ int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** o
     if (c.tuning.waves_per_simd != 0 && c.tuning.waves_per_simd != 8)
         return fail(XD_ERR_INVALID_ARGUMENT, "waves_per_simd must be 0 or 8");
     if (c.tuning.class_kernels > 1) return fail(XD_ERR_INVALID_ARGUMENT, "class_kernels must be 0 or 1");
+    if (!(c.max_wait_ms >= 0)) return fail(XD_ERR_INVALID_ARGUMENT, "max_wait_ms must be >= 0");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
     auto* s = new xd_server();
@@ -653,6 +696,15 @@ int xd_server_poll(xd_server* s, xd_completion* out, uint32_t max, uint32_t* n, 
         out[(*n)++] = s->done.front();
         s->done.pop_front();
     }
+    return XD_OK;
+}
+
+int xd_server_set_max_wait(xd_server* s, double max_wait_ms) {
+    if (!s) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    if (!(max_wait_ms >= 0)) return fail(XD_ERR_INVALID_ARGUMENT, "max_wait_ms must be >= 0");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->cfg.max_wait_ms = max_wait_ms;
+    s->cv_work.notify_one();
     return XD_OK;
 }
 
