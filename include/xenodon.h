@@ -245,6 +245,21 @@ XD_API int xd_batch_autotune(xd_batch* batch, uint32_t trials, xd_batch_tuning* 
  * whole launch; *rays the number of pixels rendered. */
 XD_API int xd_batch_render(xd_batch* batch, double* ms, uint64_t* rays);
 XD_API int xd_batch_free(xd_batch* batch);
+/* Allocation-free batched render.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Renders `count` views
+ * (volumes and frames of ctx, as xd_batch_set_views takes them) in one
+ * launch of the batched kernels under `tuning` (NULL = the defaults of a new
+ * batch) and waits; *ms is the HIP-event time of the launch, *rays the pixels
+ * rendered.  Frames are bit-identical to xd_batch_render of the same views.
+ * The view and tile tables and the queue counters live in a static device
+ * array of about 8.2 MB per GPU, held until the process ends: at most 256
+ * views and 2^21 wave tiles (64 pixels each) per call, else
+ * XD_ERR_INVALID_ARGUMENT.  It calls no hipMalloc or hipFree and captures no
+ * graph.  Calls on one GPU share that array: calls that overlap on one GPU,
+ * from any context or thread, race and must be serialised by the caller. */
+XD_API int xd_batch_render_static(xd_context* ctx, const xd_view* views, uint32_t count,
+                                  const xd_batch_tuning* tuning, double* ms, uint64_t* rays);
 
 /* ---- served form: request loop over resident volumes -------------------
  *

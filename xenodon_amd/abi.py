@@ -148,6 +148,8 @@ def _declare(L):
         "xd_batch_autotune": ([vp, ctypes.c_uint32, P(BatchTuning)], ctypes.c_int),
         "xd_batch_render": ([vp, P(ctypes.c_double), P(ctypes.c_uint64)], ctypes.c_int),
         "xd_batch_free": ([vp], ctypes.c_int),
+        "xd_batch_render_static": ([vp, P(View), ctypes.c_uint32, P(BatchTuning), P(ctypes.c_double),
+                                    P(ctypes.c_uint64)], ctypes.c_int),
         "xd_volume_footprint": ([vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                  ctypes.c_uint64, P(ctypes.c_uint64)], ctypes.c_int),
         "xd_server_create": ([vp, P(ServerConfig), P(vp)], ctypes.c_int),
@@ -363,6 +365,18 @@ def batch_tuning(tile_w, tiles_per_grab=1, waves_per_simd=8, class_kernels=0, in
     t.interleave = interleave
     t.mix_classes = mix_classes
     return t
+
+
+def render_static(ctx, views, tuning=None):
+    """xd_batch_render_static: one batched launch of `views` whose tables live
+    in the library's static device array (no device allocation).  tuning: a
+    dict of xd_batch_tuning fields, or None for the defaults.  Returns (ms, rays)."""
+    ms = ctypes.c_double()
+    rays = ctypes.c_uint64()
+    t = None if tuning is None else ctypes.byref(batch_tuning(**tuning))
+    check(lib().xd_batch_render_static(ctx.h, view_array(views), len(views), t, ctypes.byref(ms),
+                                       ctypes.byref(rays)))
+    return ms.value, rays.value
 
 
 class Batch:

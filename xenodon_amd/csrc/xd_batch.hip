@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <initializer_list>
+#include <mutex>
 #include <cstring>
 #include <vector>
 
@@ -654,6 +655,110 @@ int xd_batch_autotune(xd_batch* b, uint32_t trials, xd_batch_tuning* chosen) {
     }
     if ((rc = prepare(b))) return rc;
     if (chosen) *chosen = b->tuning;
+    return XD_OK;
+}
+
+// ---- allocation-free batched render ---------------------------------------
+//
+// The view and tile tables and the queue counters of xd_batch_render_static
+// live in one file-scope device array instead of memory the library
+// allocates.  In host code `scratch` only names the symbol: its device address
+// (one per GPU, each GPU holding its own copy) comes from hipGetSymbolAddress,
+// taken once per device with that device current, and only that address is
+// passed to copies and kernels.
+
+}  // extern "C"
+
+namespace xd {
+namespace {
+
+constexpr uint32_t kStaticViews = 256;
+constexpr uint32_t kStaticTiles = 1u << 21;  // 2M wave tiles: 134M rays at 64 per tile
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t kScratchQbeg = align256(sizeof(BatchView) * kStaticViews);
+constexpr size_t kScratchCounters = kScratchQbeg + align256(sizeof(uint32_t) * (kClasses * kQueues + 1));
+constexpr size_t kScratchTiles = kScratchCounters + align256(sizeof(uint32_t) * kClasses * kQueues);
+constexpr size_t kScratchBytes = kScratchTiles + sizeof(uint32_t) * kStaticTiles;
+static_assert(alignof(BatchView) <= 256, "view table alignment");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch[kMaxDevices] = {};
+
+}  // namespace
+}  // namespace xd
+
+static __device__ __align__(256) uint8_t scratch[xd::kScratchBytes];
+
+namespace xd {
+namespace {
+
+// Device address of this device's `scratch` (the calling thread has made
+// `device` current).
+int scratch_address(int device, uint8_t** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(XD_ERR_INVALID_ARGUMENT, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return fail_hip(e, "hipGetSymbolAddress(scratch)");
+        g_scratch[device] = static_cast<uint8_t*>(ptr);
+    }
+    *out = g_scratch[device];
+    return XD_OK;
+}
+
+}  // namespace
+}  // namespace xd
+
+extern "C" {
+
+int xd_batch_render_static(xd_context* ctx, const xd_view* views, uint32_t count, const xd_batch_tuning* tuning,
+                           double* ms, uint64_t* rays) {
+    if (!ctx || (!views && count)) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
+    if (count > kStaticViews)
+        return fail(XD_ERR_INVALID_ARGUMENT, "%u views exceed the static limit of %u", count, kStaticViews);
+    const xd_batch_tuning tun = tuning ? *tuning : default_tuning();
+    int rc = check_tuning(&tun);
+    if (rc) return rc;
+    if ((rc = check_views(ctx, views, count))) return rc;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    uint8_t* base = nullptr;
+    if ((rc = scratch_address(ctx->device, &base))) return rc;
+    HostTables T;
+    if ((rc = build_tables(views, count, tun, T))) return rc;
+    if (rays) *rays = T.total_rays;
+    if (ms) *ms = 0;
+    if (count == 0) return XD_OK;
+    const uint32_t total = (uint32_t)T.tiles.size();
+    if (total > kStaticTiles)
+        return fail(XD_ERR_INVALID_ARGUMENT, "%u wave tiles exceed the static limit of %u", total, kStaticTiles);
+    const bool w8 = tun.waves_per_simd == 8;
+    BatchClasses cls = T.cls;
+    for (int c = 0; c <= (int)kClasses; ++c)
+        if (cls.grid[c] && (rc = batch_grid(ctx->device, c, cls.slots[c], w8, &cls.grid[c]))) return rc;
+    BatchView* d_views = reinterpret_cast<BatchView*>(base);
+    uint32_t* d_qbeg = reinterpret_cast<uint32_t*>(base + kScratchQbeg);
+    uint32_t* d_counters = reinterpret_cast<uint32_t*>(base + kScratchCounters);
+    uint32_t* d_tiles = reinterpret_cast<uint32_t*>(base + kScratchTiles);
+    hipStream_t s = ctx->stream;
+    // launch_batch clears the counters with a kernel before the class kernels
+    if ((e = hipMemcpyAsync(d_views, T.views.data(), sizeof(BatchView) * count, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (total && (e = hipMemcpyAsync(d_tiles, T.tiles.data(), sizeof(uint32_t) * total, hipMemcpyHostToDevice, s)) != hipSuccess) ||
+        (e = hipMemcpyAsync(d_qbeg, T.qbeg, sizeof(T.qbeg), hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipEventRecord(ctx->ev0, s)) != hipSuccess ||
+        (e = launch_batch(d_views, d_tiles, d_qbeg, d_counters, tun.tiles_per_grab, cls, w8, tun.mix_classes != 0, s)) !=
+            hipSuccess ||
+        (e = hipEventRecord(ctx->ev1, s)) != hipSuccess || (e = hipEventSynchronize(ctx->ev1)) != hipSuccess) {
+        (void)hipStreamSynchronize(s);  // no copy may still read the host tables when they go
+        return fail_hip(e, "static batch render");
+    }
+    if (ms) {
+        float f = 0;
+        if ((e = hipEventElapsedTime(&f, ctx->ev0, ctx->ev1)) != hipSuccess) return fail_hip(e, "hipEventElapsedTime");
+        *ms = f;
+    }
     return XD_OK;
 }
 
