@@ -215,6 +215,7 @@ struct xd_batch {
     hipGraphExec_t exec = nullptr;
     BatchClasses graph_cls{};  // per-class grids and LDS stack strides the graph was captured with
     uint32_t graph_grab = 0, graph_w8 = 0, graph_mix = 0;
+    bool keep_mixed = false;  // autotuner: one traversal's views on the all-traversal kernel
 };
 
 namespace {
@@ -305,7 +306,11 @@ struct HostTables {
 // Builds the view and tile tables of n views under a tuning.  Tiles are
 // grouped by class, and within a class dealt into kQueues ranges by the band
 // of their view they lie in.
-int build_tables(const xd_view* views, uint32_t n, const xd_batch_tuning& tuning, HostTables& T) {
+// keep_mixed: a batch of one traversal still runs the all-traversal kernel
+// when the tuning says so (the autotuner times one traversal's views on the
+// kernel the whole batch will run).
+int build_tables(const xd_view* views, uint32_t n, const xd_batch_tuning& tuning, HostTables& T,
+                 bool keep_mixed = false) {
     T.views.assign(n, BatchView{});
     std::vector<BatchView>& bv = T.views;
     std::vector<std::vector<uint32_t>> q(kClasses * kQueues);
@@ -373,7 +378,7 @@ int build_tables(const xd_view* views, uint32_t n, const xd_batch_tuning& tuning
     // that loads the next steps' macro words during the current gathers).
     uint32_t present = 0;
     for (uint32_t c = 0; c < kClasses; ++c) present += cls.grid[c] ? 1u : 0u;
-    const bool per_class = tuning.class_kernels || present == 1;
+    const bool per_class = tuning.class_kernels || (present == 1 && !keep_mixed);
     // one LDS layout per kernel: every wave's stack region is sized for the
     // deepest tree among the views that kernel drains
     if (!per_class) {
@@ -409,7 +414,7 @@ int build_tables(const xd_view* views, uint32_t n, const xd_batch_tuning& tuning
 int upload_tables(xd_batch* b, BatchClasses* need) {
     HostTables T;
     const uint32_t n = (uint32_t)b->views.size();
-    int rc = build_tables(b->views.data(), n, b->tuning, T);
+    int rc = build_tables(b->views.data(), n, b->tuning, T, b->keep_mixed);
     if (rc) return rc;
     b->total_rays = T.total_rays;
     const uint32_t total = (uint32_t)T.tiles.size();
@@ -631,7 +636,17 @@ int xd_batch_autotune(xd_batch* b, uint32_t trials, xd_batch_tuning* chosen) {
     const std::vector<xd_view> all = b->views;
     const xd_batch_tuning start = b->tuning;
     int rc = XD_OK;
-    // tile shape per traversal, timed on that traversal's views alone
+    // tile shape per traversal, timed on that traversal's views alone, on
+    // the kernel the whole batch runs (a batch of one traversal would
+    // otherwise run that traversal's own kernel)
+    uint32_t present = 0;
+    for (int t = 0; t < 5; ++t)
+        for (const auto& v : all)
+            if (v.traversal == t) {
+                ++present;
+                break;
+            }
+    b->keep_mixed = present > 1 && !b->tuning.class_kernels;
     for (int t = 0; t < 5 && !rc; ++t) {
         std::vector<xd_view> sub;
         for (const auto& v : all)
@@ -640,6 +655,7 @@ int xd_batch_autotune(xd_batch* b, uint32_t trials, xd_batch_tuning* chosen) {
         b->views = sub;
         rc = sweep(b, trials, &b->tuning.tile_w[t], {8u, 16u, 32u});
     }
+    b->keep_mixed = false;
     // then the launch shape, on the whole batch, one field at a time from
     // the starting tuning
     b->views = all;
