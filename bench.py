@@ -261,20 +261,20 @@ def load_served(srv, volumes):
 
 
 def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plain=False, server=None,
-               max_wait_ms=0.0):
+               latency_target_ms=0.0):
     """Poisson arrivals at `rate` req/s for `duration_s` (or n_saturated
     requests all released at once).  volumes: name -> (kind, data, model_dim).
     Returns latency percentiles (ms) and aggregate Mrays/s of the requests
     served; with plain, also the plain form's rate on the same requests.
     server = (server, ids) from load_served reuses resident volumes; otherwise
-    a server is made for the leg.  max_wait_ms: the server's latency target
+    a server is made for the leg.  latency_target_ms: the server's latency target
     for this leg (requests that cannot start within it are shed and counted)."""
     if server is None:
-        srv = ctx.server(tuning=tuning, max_wait_ms=max_wait_ms, **SERVER_CONFIG)
+        srv = ctx.server(tuning=tuning, latency_target_ms=latency_target_ms, **SERVER_CONFIG)
         ids = load_served(srv, volumes)
     else:
         srv, ids = server
-        srv.set_max_wait(max_wait_ms)
+        srv.set_latency_target(latency_target_ms)
     st0 = srv.stats()
     n = n_saturated if n_saturated else int(rate * duration_s)
     named, rng = workloads.request_mix({k: v[1] for k, v in ids.items()}, n, seed)
@@ -301,15 +301,15 @@ def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plai
     if server is None:
         srv.free()
     if server is not None:
-        srv.set_max_wait(0.0)
+        srv.set_latency_target(0.0)
     shed = [c for c in done if c["status"] == abi.XD_ERR_REJECTED]
     bad = [c for c in done if c["status"] not in (abi.XD_OK, abi.XD_ERR_REJECTED)]
-    if bad or (shed and not max_wait_ms):
+    if bad or (shed and not latency_target_ms):
         c = (bad or shed)[0]
         raise RuntimeError(f"{len(bad) + len(shed)} served requests failed (status {c['status']})")
     done = [c for c in done if c["status"] == abi.XD_OK]
     if not done:
-        raise RuntimeError(f"all {n} requests were shed under a {max_wait_ms} ms target")
+        raise RuntimeError(f"all {n} requests were shed under a {latency_target_ms} ms target")
     lat = np.array([c["done_ms"] - c["arrival_ms"] for c in done])
     dev = np.array([c["device_done_ms"] - c["arrival_ms"] for c in done])
     wall = max(c["done_ms"] for c in done) - min(c["arrival_ms"] for c in done)
@@ -323,8 +323,8 @@ def served_leg(ctx, volumes, rate, duration_s, seed, tuning, n_saturated=0, plai
            # summed launch durations over the wall time: launches overlap on
            # the device, so this can exceed 1 (it is not a busy fraction)
            "launch_ms_sum_over_wall": round((st["busy_ms"] - st0["busy_ms"]) / wall, 3)}
-    if max_wait_ms:
-        res["max_wait_ms"] = max_wait_ms
+    if latency_target_ms:
+        res["latency_target_ms"] = latency_target_ms
         res["served"] = len(done)
         res["shed"] = len(shed)
         res["shed_frac"] = round(len(shed) / n, 4)
@@ -401,7 +401,7 @@ def main():
                     help="also serve requests over the 2048^3 and 1024^3 log-normal volumes next to the others")
     ap.add_argument("--c5-rates", default="50,200,500,1000,2000")
     ap.add_argument("--c5-seconds", type=float, default=10.0)
-    ap.add_argument("--c5-max-wait-ms", type=float, default=100.0,
+    ap.add_argument("--c5-latency-target-ms", type=float, default=100.0,
                     help="latency target of the 2048^3 served legs run a second time with shedding (0: skip)")
     ap.add_argument("--c5-target-rates", default="1000,2000")
     ap.add_argument("--serve-saturated", type=int, default=400,
@@ -735,13 +735,13 @@ def main():
             res = served_leg(ctx, c5vols, r, args.c5_seconds, seed=2000 + k, tuning=serve_tuning, server=(srv, ids))
             log(f"served (2048^3 resident) @ {r:.0f} req/s: {res['mrays_s']} Mrays/s, p99 {res['p99_ms']} ms")
             served_c5.append(res)
-        if args.c5_max_wait_ms > 0:
+        if args.c5_latency_target_ms > 0:
             # the same arrivals with a latency target: what cannot start within
             # it is shed instead of queued
             for k, r in enumerate(float(x) for x in args.c5_target_rates.split(",") if x):
                 res = served_leg(ctx, c5vols, r, args.c5_seconds, seed=2100 + k, tuning=serve_tuning,
-                                 server=(srv, ids), max_wait_ms=args.c5_max_wait_ms)
-                log(f"served (2048^3 resident, target {args.c5_max_wait_ms:.0f} ms) @ {r:.0f} req/s: "
+                                 server=(srv, ids), latency_target_ms=args.c5_latency_target_ms)
+                log(f"served (2048^3 resident, target {args.c5_latency_target_ms:.0f} ms) @ {r:.0f} req/s: "
                     f"{res['mrays_s']} Mrays/s, p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms, "
                     f"shed {res['shed_frac']:.1%}")
                 served_c5.append(res)

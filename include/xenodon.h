@@ -289,14 +289,14 @@ typedef struct {
     xd_batch_tuning tuning;      /* tile shapes etc.; zero-filled = defaults.  A
                                     step whose tiles are all of one traversal
                                     runs that traversal's own kernel */
-    double max_wait_ms;          /* latency target; 0 = none (every request waits
-                                    for its turn).  With a target, a request is
-                                    admitted only while the rays the admitted
-                                    requests still have to render take at most
-                                    max_wait_ms at the device's measured rate,
-                                    and a request that has waited max_wait_ms
-                                    for admission is shed: it completes with
-                                    status XD_ERR_REJECTED and no frame */
+    double latency_target_ms;    /* 0 = none (every request waits for its turn).
+                                    With a target, a request coming up for
+                                    admission whose arrival-to-done time,
+                                    estimated from the rays the admitted
+                                    requests still have to render, its own and
+                                    the device's measured rate, exceeds the
+                                    target is shed: it completes with status
+                                    XD_ERR_REJECTED and no frame */
 } xd_server_config;
 
 typedef struct {
@@ -330,7 +330,7 @@ typedef struct {
     uint64_t completed, launches, tiles, rays;
     uint32_t max_requests_per_launch, max_launches_overlapped;
     double busy_ms;              /* sum of launch durations (HIP events) */
-    uint64_t shed;               /* requests shed under max_wait_ms */
+    uint64_t shed;               /* requests shed under latency_target_ms */
     double rays_per_ms;          /* measured device rate the admission uses */
 } xd_server_stats;
 
@@ -351,9 +351,9 @@ XD_API int xd_server_submit(xd_server* srv, const xd_request* req, uint64_t* req
  * (0 = do not wait, <0 = wait for at least one). */
 XD_API int xd_server_poll(xd_server* srv, xd_completion* out, uint32_t max, uint32_t* n, int32_t timeout_ms);
 XD_API int xd_server_stats_get(xd_server* srv, xd_server_stats* out);
-/* Changes the latency target (xd_server_config.max_wait_ms) of a running
+/* Changes the latency target (xd_server_config.latency_target_ms) of a running
  * server; it applies from the next admission on. */
-XD_API int xd_server_set_max_wait(xd_server* srv, double max_wait_ms);
+XD_API int xd_server_set_latency_target(xd_server* srv, double latency_target_ms);
 /* Current time on the server clock (ms). */
 XD_API double xd_server_now_ms(const xd_server* srv);
 /* Finishes every queued request, stops the threads, frees the volumes. */

@@ -14,7 +14,9 @@
 //     the node reached by resuming the walk over the tagged words from the
 //     table entry of the point's level-6 cell, with the same offset and side;
 //   * rope depths (when packed): bits 25..29 of every leaf child word hold the
-//     depth field of the node it names.
+//     depth field of the node it names;
+//   * the device form (to_rope_planes): the words of the nodes before the first
+//     leaf unchanged, and rope k of every leaf at its place in plane k.
 // Prints "ok" and exits 0, or prints the first failures and exits 1.
 #include <cmath>
 #include <cstdint>
@@ -149,8 +151,8 @@ int main(int argc, char** argv) {
     const bool tree = n >= (1ull << 22) && octree_is_tree(w.data(), n);
     std::vector<uint32_t> cw(n * 8), meta(n * 2);
     uint32_t top[kTopLevels];
-    uint32_t empty_start = 0;
-    build_tagged_layout(w.data(), n, cw.data(), meta.data(), top, wide, tree, &empty_start);
+    uint32_t empty_start = 0, leaf_base = 0;
+    build_tagged_layout(w.data(), n, cw.data(), meta.data(), top, wide, tree, &empty_start, &leaf_base);
     std::vector<uint32_t> id = reference_ids(w.data(), n);
     {
         // internal nodes (and the root), then leaves with colour, then empty
@@ -167,6 +169,7 @@ int main(int argc, char** argv) {
         const uint32_t kept = count[0] + count[1];
         CHECK(empty_start == (wide ? kept : 0xFFFFFFFFu), "empty_start %u (wide %d, %u non-empty)", empty_start,
               (int)wide, kept);
+        CHECK(leaf_base == count[0], "leaf_base %u, %u internal nodes and root", leaf_base, count[0]);
         uint32_t at[3] = {0, count[0], kept};
         for (uint64_t k = 0; k < n; ++k) id[order[k]] = at[cls(order[k])]++;
     }
@@ -201,6 +204,19 @@ int main(int argc, char** argv) {
                       "rope %d of leaf %llu", k, (unsigned long long)i);
             }
         }
+
+    // the device form: ropes of the leaves as planes
+    {
+        std::vector<uint32_t> planes(rope_plane_words(n, leaf_base));
+        to_rope_planes(cw.data(), n, leaf_base, planes.data());
+        CHECK(planes.size() == (uint64_t)leaf_base * 8 + 6 * (n - leaf_base), "plane words %zu", planes.size());
+        CHECK(std::memcmp(planes.data(), cw.data(), (uint64_t)leaf_base * 8 * sizeof(uint32_t)) == 0,
+              "words before the first leaf");
+        for (uint64_t l = leaf_base; l < n; ++l)
+            for (uint64_t k = 0; k < 6; ++k)
+                CHECK(planes[(uint64_t)leaf_base * 8 + k * (n - leaf_base) + (l - leaf_base)] == cw[l * 8 + k],
+                      "rope %llu of leaf %llu in its plane", (unsigned long long)k, (unsigned long long)l);
+    }
 
     const uint32_t root_tag =
         (w[9] & kLeafMask) ? (kTagLeaf | ((w[8] & 0xFFFFFFu) ? 0u : empty_tag)) : 0u;

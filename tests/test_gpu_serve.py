@@ -218,15 +218,16 @@ def test_free_finishes_queued_requests(env):
         assert np.array_equal(img, ref)
 
 
-def test_latency_target_sheds_what_cannot_start_in_time(env):
-    """With max_wait_ms set, a burst far beyond what the device renders in that
-    time is partly shed: shed requests complete with XD_ERR_REJECTED and leave
-    their host buffer untouched, every admitted request started within the
-    target (plus the worker's scheduling jitter), and its frame still equals
-    the plain render."""
+def test_latency_target_sheds_what_cannot_finish_in_time(env):
+    """With latency_target_ms set, a burst far beyond what the device renders in
+    that time is partly shed: shed requests complete promptly with
+    XD_ERR_REJECTED and leave their host buffer untouched, every admitted
+    request is done within the target (plus readback and the worker's
+    scheduling jitter; without a target the last of this burst would take
+    about 250 ms), and its frame still equals the plain render."""
     ctx, data, plain = env
     target, W, Hh = 4.0, 1920, 1080  # 120 such frames are about 250 M rays: far more than 4 ms of work
-    srv = ctx.server(max_wait_ms=target)
+    srv = ctx.server(latency_target_ms=target)
     ids = _load_all(srv, data)
     cams = H.orbit_cameras(120, 1.8, seed=12)
     kept = {}
@@ -243,14 +244,14 @@ def test_latency_target_sheds_what_cannot_start_in_time(env):
     assert st["shed"] == len(shed) and st["rays_per_ms"] > 0
     for c in shed:
         assert not kept[c["tag"]].any()
-        assert c["done_ms"] - c["arrival_ms"] >= target
+        assert c["done_ms"] - c["arrival_ms"] <= target + 20.0
     for c in served:
-        assert c["start_ms"] - c["arrival_ms"] <= target + 20.0
+        assert c["done_ms"] - c["arrival_ms"] <= target + 20.0
         ref = _plain(ctx, plain, "shells_svo", abi.SVO_DF, cams[c["tag"]], W, Hh, 2.5)
         assert np.array_equal(kept[c["tag"]], ref)
 
 
 def test_bad_latency_target_is_refused(env):
     ctx, data, plain = env
-    with pytest.raises(abi.XdError, match="max_wait_ms"):
-        ctx.server(max_wait_ms=-1.0)
+    with pytest.raises(abi.XdError, match="latency_target_ms"):
+        ctx.server(latency_target_ms=-1.0)

@@ -82,7 +82,7 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("frame_arena_bytes", ctypes.c_uint64),
                 ("max_active", ctypes.c_uint32), ("tiles_per_launch", ctypes.c_uint32),
                 ("launches_in_flight", ctypes.c_uint32), ("readback_threads", ctypes.c_uint32),
-                ("tuning", BatchTuning), ("max_wait_ms", ctypes.c_double)]
+                ("tuning", BatchTuning), ("latency_target_ms", ctypes.c_double)]
 
 
 class Request(ctypes.Structure):
@@ -153,7 +153,7 @@ def _declare(L):
         "xd_volume_footprint": ([vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
                                  ctypes.c_uint64, P(ctypes.c_uint64)], ctypes.c_int),
         "xd_server_create": ([vp, P(ServerConfig), P(vp)], ctypes.c_int),
-        "xd_server_set_max_wait": ([vp, ctypes.c_double], ctypes.c_int),
+        "xd_server_set_latency_target": ([vp, ctypes.c_double], ctypes.c_int),
         "xd_server_load_grid": ([vp, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_uint32)],
                                 ctypes.c_int),
         "xd_server_load_svo": ([vp, vp, ctypes.c_uint64, ctypes.c_uint64, P(ctypes.c_uint32)], ctypes.c_int),
@@ -280,9 +280,9 @@ class Context:
         return self._adopt(Batch(self, h))
 
     def server(self, hbm_budget_bytes=0, frame_arena_bytes=0, max_active=0, tiles_per_launch=0,
-               launches_in_flight=0, readback_threads=0, tuning=None, max_wait_ms=0.0):
+               launches_in_flight=0, readback_threads=0, tuning=None, latency_target_ms=0.0):
         c = ServerConfig()
-        c.max_wait_ms = float(max_wait_ms)
+        c.latency_target_ms = float(latency_target_ms)
         c.hbm_budget_bytes = hbm_budget_bytes
         c.frame_arena_bytes = frame_arena_bytes
         c.max_active = max_active
@@ -488,9 +488,9 @@ class Server:
                         "done_ms": c.done_ms, "rays": c.rays, "image": self._dst.pop(c.request_id, None)})
         return out
 
-    def set_max_wait(self, ms):
+    def set_latency_target(self, ms):
         """Latency target (ms) for admissions from now on; 0 = none."""
-        check(lib().xd_server_set_max_wait(self.h, float(ms)))
+        check(lib().xd_server_set_latency_target(self.h, float(ms)))
 
     def stats(self):
         st = ServerStats()

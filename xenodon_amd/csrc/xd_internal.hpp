@@ -78,6 +78,8 @@ struct RenderParams {
     uint32_t index_mask;   // index bits of a tagged child word (kIndexMask, kWideIndexMask, kRopeIndexMask)
     uint32_t empty_tag;    // kTagEmpty, or 0 for wide tags
     uint32_t empty_start;  // wide tags: child indices from here on are empty leaves (else 0xFFFFFFFF)
+    uint32_t leaf_base;    // tagged layout: first leaf; rope k of leaf l >= leaf_base is word
+                           // leaf_base*8 + k*(node_count - leaf_base) + (l - leaf_base) of cw
     uint32_t xcd_chunks;  // plain launches: block runs per XCD (0: hardware order)
     unsigned long long* words;  // word counter (instrumented variant only)
 };

@@ -19,6 +19,11 @@
 // Trees above 2^30 nodes use 31-bit indices and no empty tag: an index at or
 // past the first empty leaf's marks one.
 // meta[i] = {color, is_leaf_depth} of node i.
+// On the device the leaves' child words -- their ropes, which only svo-rope
+// reads, one word per leaf it crosses -- are stored as six planes
+// (to_rope_planes): rope k of leaf leaf_base + j at leaf_base*8 + k*n_leaves + j,
+// so the ropes of consecutive leaves share cache lines (32 per line instead
+// of the 4 leaves' 32-byte records of the grouped layout).
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
@@ -115,7 +120,7 @@ uint32_t number_subtrees(const uint32_t* w, const std::vector<uint32_t>& roots, 
 // top[] becomes the count of internal nodes of the top levels (whose child
 // words the rope kernel stages in LDS).
 void partition_leaves(const uint32_t* w, uint64_t n, uint32_t* new_id, uint32_t top[kTopLevels],
-                            uint32_t* empty_start) {
+                            uint32_t* empty_start, uint32_t* leaf_base) {
     // class 0: internal nodes (and the root), 1: leaves with colour, 2: empty leaves
     auto cls = [w](uint64_t i) -> int {
         const uint32_t* nd = w + i * kNodeWords;
@@ -155,12 +160,13 @@ void partition_leaves(const uint32_t* w, uint64_t n, uint32_t* new_id, uint32_t 
         top[L] = c;
     }
     *empty_start = (uint32_t)base[2];
+    *leaf_base = (uint32_t)base[1];
 }
 
 }  // namespace
 
 void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels],
-                         bool wide, bool tree, uint32_t* empty_start) {
+                         bool wide, bool tree, uint32_t* empty_start, uint32_t* leaf_base) {
     const uint32_t empty_tag = wide ? 0u : kTagEmpty;
     std::vector<uint32_t> new_id(n, 0xFFFFFFFFu);
     uint32_t next = 1;
@@ -215,7 +221,7 @@ void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint3
     // tags the empty leaves' indices mark them; with narrow tags the tag does,
     // and the order still packs the descents' child words together
     // (svo-naive +4 %, esvo +2 % on the 512x361x512 tree)
-    partition_leaves(w, n, new_id.data(), top, empty_start);
+    partition_leaves(w, n, new_id.data(), top, empty_start, leaf_base);
     if (!wide) *empty_start = 0xFFFFFFFFu;
 
     const unsigned T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -270,6 +276,21 @@ bool pack_rope_depths(uint32_t* child_w, const uint32_t* meta, uint64_t n) {
         }
     }
     return true;
+}
+
+uint64_t rope_plane_words(uint64_t n, uint32_t leaf_base) { return (uint64_t)leaf_base * 8 + 6 * (n - leaf_base); }
+
+void to_rope_planes(const uint32_t* child_w, uint64_t n, uint32_t leaf_base, uint32_t* out) {
+    const uint64_t nl = n - leaf_base, base = (uint64_t)leaf_base * 8;
+    std::memcpy(out, child_w, base * sizeof(uint32_t));
+    const unsigned T = worker_count();
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; ++t)
+        th.emplace_back([&, t] {
+            for (uint64_t j = nl * t / T; j < nl * (t + 1) / T; ++j)
+                for (uint64_t k = 0; k < 6; ++k) out[base + k * nl + j] = child_w[(leaf_base + j) * 8 + k];
+        });
+    for (auto& x : th) x.join();
 }
 
 void build_start_table(const uint32_t* child_w, uint32_t root_tag, uint32_t index_mask, uint2* table) {

@@ -120,6 +120,7 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
         p.lds_slots = p.stack_slots;
         p.node_count = (uint32_t)vol->node_count;
         p.depth_exact = vol->depth_exact;
+        p.leaf_base = vol->leaf_base;
     }
 }
 
@@ -466,14 +467,24 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
         const bool wide = node_count > kMaxNarrowTagNodes || ctx->octree_layout == 2;
         // large trees (the converter's, not DAGs) are numbered and checked in parallel
         const bool tree = node_count >= (1ull << 22) && octree_is_tree(w, node_count);
-        build_tagged_layout(w, node_count, cw, meta, v->top, wide, tree, &v->empty_start);
+        build_tagged_layout(w, node_count, cw, meta, v->top, wide, tree, &v->empty_start, &v->leaf_base);
         v->depth_exact = octree_depths_exact(w, node_count, tree) ? 1u : 0u;
         v->rope_depth = !wide && pack_rope_depths(cw, meta, node_count) ? 1u : 0u;
         v->index_mask = v->rope_depth ? kRopeIndexMask : wide ? kWideIndexMask : kIndexMask;
         v->empty_tag = wide ? 0u : kTagEmpty;
+        // the leaves' ropes as planes (the allocation keeps the grouped size,
+        // which the footprint states before the tree is seen)
+        const uint64_t cw_words = rope_plane_words(node_count, v->leaf_base);
+        std::unique_ptr<uint32_t[]> planes(new (std::nothrow) uint32_t[cw_words]);
+        if (!planes) {
+            xd_volume_free(v);
+            return fail(XD_ERR_OUT_OF_MEMORY, "out of host memory deriving the octree layout");
+        }
+        to_rope_planes(cw, node_count, v->leaf_base, planes.get());
+        cw_buf.reset();
         if ((e = hipMalloc(&v->d_cw, node_count * 8 * sizeof(uint32_t))) != hipSuccess ||
             (e = hipMalloc(&v->d_meta, node_count * sizeof(uint2))) != hipSuccess ||
-            (e = hipMemcpy(v->d_cw, cw, node_count * 8 * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
+            (e = hipMemcpy(v->d_cw, planes.get(), cw_words * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
                 hipSuccess ||
             (e = hipMemcpy(v->d_meta, meta, node_count * sizeof(uint2), hipMemcpyHostToDevice)) != hipSuccess) {
             xd_volume_free(v);
@@ -482,7 +493,7 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
         const uint32_t root_ld = w[9], root_col = w[8];
         v->root_tag = (root_ld & kLeafMask) ? (kTagLeaf | ((root_col & 0xFFFFFFu) ? 0u : v->empty_tag)) : 0u;
         std::vector<uint2> table(kTableCells);
-        build_start_table(cw, v->root_tag, v->index_mask, table.data());
+        build_start_table(planes.get(), v->root_tag, v->index_mask, table.data());  // internal nodes' words as in cw
         if ((e = hipMalloc(&v->d_table, kTableBytes)) != hipSuccess ||
             (e = hipMemcpy(v->d_table, table.data(), kTableBytes, hipMemcpyHostToDevice)) != hipSuccess) {
             xd_volume_free(v);

@@ -54,6 +54,7 @@ struct xd_volume {
     uint32_t root_tag = 0;
     uint32_t depth_exact = 0;  // internal nodes' depth fields match their depth (see octree_depths_exact)
     uint32_t top[xd::kTopLevels] = {0, 0, 0, 0};  // nodes in breadth-first levels 0..L
+    uint32_t leaf_base = 0;    // tagged layout: first leaf; the leaves' ropes are stored as planes
 };
 
 struct xd_frame {
@@ -85,8 +86,14 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
 // first of them, which is how a child word marks an empty leaf (else
 // *empty_start = 0xFFFFFFFF and the empty tag does).  tree: octree_is_tree(w, n) holds, so the
 // numbering runs subtree-parallel.
+// *leaf_base: the first leaf's index (internal nodes and the root come first).
 void build_tagged_layout(const uint32_t* w, uint64_t n, uint32_t* child_w, uint32_t* meta, uint32_t top[kTopLevels],
-                         bool wide, bool tree, uint32_t* empty_start);
+                         bool wide, bool tree, uint32_t* empty_start, uint32_t* leaf_base);
+// The device form of child_w: the words of nodes [0, leaf_base) as they are,
+// then the six ropes of the leaves as planes, rope k of leaf leaf_base + j at
+// leaf_base*8 + k*(n - leaf_base) + j (rope_plane_words words in all).
+uint64_t rope_plane_words(uint64_t n, uint32_t leaf_base);
+void to_rope_planes(const uint32_t* child_w, uint64_t n, uint32_t leaf_base, uint32_t* out);
 // True when no internal node's child word names the root and no node is
 // named by two child words of internal nodes (every node has one parent).
 bool octree_is_tree(const uint32_t* w, uint64_t n);

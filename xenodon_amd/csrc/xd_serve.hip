@@ -199,17 +199,21 @@ void abort_all(xd_server* s, int status) {
 }
 
 // Admits released requests oldest first while slots and frame memory last.
-// Under a latency target (max_wait_ms > 0) a request is also held back while
-// the admitted requests' remaining rays would take longer than the target at
-// the measured rate, and a request that has waited longer than the target is
-// shed.  Until a first step has been measured the rate is unknown, and one
-// request at a time is admitted.
+// Under a latency target (latency_target_ms > 0) a request's arrival-to-done
+// time is estimated when it comes up for admission: the wait so far plus the
+// rays the admitted requests still have to render and its own, at the
+// measured rate (all admitted requests progress together).  A request whose
+// estimate exceeds the target is shed; every other one is admitted, so the
+// device stays busy with the work that can still meet the target.  Until a
+// first step has been measured the rate is unknown, and one request at a time
+// is admitted.
 void admit(xd_server* s, double now) {
-    const double target = s->cfg.max_wait_ms;
+    const double target = s->cfg.latency_target_ms;
     for (auto it = s->pending.begin(); it != s->pending.end();) {
         Request* q = it->second;
         if (q->c.arrival_ms > now) break;  // not released yet (and neither is any later one)
-        if (target > 0 && now - q->c.arrival_ms > target) {
+        if (target > 0 && s->rays_per_ms > 0 &&
+            now - q->c.arrival_ms + (double)(s->outstanding_rays + q->c.rays) / s->rays_per_ms > target) {
             q->c.status = XD_ERR_REJECTED;
             q->c.done_ms = now;
             s->vols[q->vol_id].refs--;
@@ -221,9 +225,7 @@ void admit(xd_server* s, double now) {
             continue;
         }
         if (s->active.size() >= s->cfg.max_active) break;
-        if (target > 0 && !s->active.empty() &&
-            (s->rays_per_ms <= 0 || (double)(s->outstanding_rays + q->c.rays) > s->rays_per_ms * target))
-            break;
+        if (target > 0 && s->rays_per_ms <= 0 && !s->active.empty()) break;
         uint64_t off = 0;
         if (!s->arena.alloc(q->frame_bytes, &off)) break;  // oldest first: wait for memory
         q->arena_off = off;
@@ -537,7 +539,7 @@ int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** o
     if (c.tuning.waves_per_simd != 0 && c.tuning.waves_per_simd != 8)
         return fail(XD_ERR_INVALID_ARGUMENT, "waves_per_simd must be 0 or 8");
     if (c.tuning.class_kernels > 1) return fail(XD_ERR_INVALID_ARGUMENT, "class_kernels must be 0 or 1");
-    if (!(c.max_wait_ms >= 0)) return fail(XD_ERR_INVALID_ARGUMENT, "max_wait_ms must be >= 0");
+    if (!(c.latency_target_ms >= 0)) return fail(XD_ERR_INVALID_ARGUMENT, "latency_target_ms must be >= 0");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
     auto* s = new xd_server();
@@ -699,11 +701,11 @@ int xd_server_poll(xd_server* s, xd_completion* out, uint32_t max, uint32_t* n, 
     return XD_OK;
 }
 
-int xd_server_set_max_wait(xd_server* s, double max_wait_ms) {
+int xd_server_set_latency_target(xd_server* s, double latency_target_ms) {
     if (!s) return fail(XD_ERR_INVALID_ARGUMENT, "null argument");
-    if (!(max_wait_ms >= 0)) return fail(XD_ERR_INVALID_ARGUMENT, "max_wait_ms must be >= 0");
+    if (!(latency_target_ms >= 0)) return fail(XD_ERR_INVALID_ARGUMENT, "latency_target_ms must be >= 0");
     std::lock_guard<std::mutex> lk(s->mu);
-    s->cfg.max_wait_ms = max_wait_ms;
+    s->cfg.latency_target_ms = latency_target_ms;
     s->cv_work.notify_one();
     return XD_OK;
 }

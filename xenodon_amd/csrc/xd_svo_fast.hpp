@@ -32,7 +32,17 @@ struct FastSvo {
     uint32_t index_mask;   // index bits of a child word
     uint32_t empty_tag;    // kTagEmpty, or 0 where leaves carry no empty tag
     uint32_t empty_start;  // ... where they do not, the empty leaves' first index
+    uint32_t leaf_base;    // first leaf: the leaves' ropes are stored as six planes
 };
+
+// Rope k of a leaf (its k-th child word): in the plane of rope k for leaves
+// from leaf_base on, else (a leaf root) in its own grouped record.
+__device__ __forceinline__ uint32_t rope_at(const FastSvo& s, uint32_t leaf, uint32_t k) {
+    const uint64_t i = leaf >= s.leaf_base
+                           ? (uint64_t)s.leaf_base * 8 + (uint64_t)k * (s.n - s.leaf_base) + (leaf - s.leaf_base)
+                           : (uint64_t)leaf * 8 + k;
+    return *(const __attribute__((address_space(1))) uint32_t*)(s.cw + i);
+}
 
 // A child word names a leaf with zero RGB (whose colour need not be loaded).
 __device__ __forceinline__ bool empty_leaf(const FastSvo& s, uint32_t wd) {
@@ -351,7 +361,7 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
     f3 total = muls(unorm_rgb(colour_if(s, node, tag)), step);
     f3 mask;
     uint32_t n = neighbor_index(nbx, nby, nbz, far, mask);
-    uint32_t wd = cw_at(s, node, n);
+    uint32_t wd = rope_at(s, node, n);
     node = wd & rope_mask;
     tag = wd;
     offset = add(offset, muls(mul(mask, sgn), side));
@@ -395,7 +405,7 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
         const uint32_t col =
             empty_leaf(s, tag) ? 0u : (node == start && !s.rope_depth ? start_col : s.meta[node].x);
         n = neighbor_index(nbx, nby, nbz, far, mask);
-        wd = cw_at(s, node, n);  // in flight together with the colour
+        wd = rope_at(s, node, n);  // in flight together with the colour
         total = add(total, muls(unorm_rgb(col), step));
         node = wd & rope_mask;
         tag = wd;

@@ -15,17 +15,24 @@ struct RawNodes {
     const uint32_t* w;
     uint32_t n;
     __device__ __forceinline__ uint32_t child(uint32_t n, uint32_t k) const { return w[(uint64_t)n * kNodeWords + k]; }
+    __device__ __forceinline__ uint32_t rope(uint32_t n, uint32_t k) const { return w[(uint64_t)n * kNodeWords + k]; }
     __device__ __forceinline__ uint32_t color(uint32_t n) const { return w[(uint64_t)n * kNodeWords + 8]; }
     __device__ __forceinline__ uint32_t leafdepth(uint32_t n) const { return w[(uint64_t)n * kNodeWords + 9]; }
 };
 
 struct TaggedNodes {
-    const uint32_t* child_w;  // node_count * 8 tagged child words
+    const uint32_t* child_w;  // tagged child words: 8 per node below leaf_base, then the leaves' ropes as planes
     const uint2* meta;        // node_count * {color, is_leaf_depth}
     uint32_t n;
     uint32_t mask;  // index bits of a child word (the volume's index_mask)
+    uint32_t leaf_base;
     __device__ __forceinline__ uint32_t child(uint32_t n, uint32_t k) const {
         return child_w[(uint64_t)n * 8 + k] & mask;
+    }
+    __device__ __forceinline__ uint32_t rope(uint32_t l, uint32_t k) const {
+        const uint64_t i = l >= leaf_base ? (uint64_t)leaf_base * 8 + (uint64_t)k * (n - leaf_base) + (l - leaf_base)
+                                          : (uint64_t)l * 8 + k;
+        return child_w[i] & mask;
     }
     __device__ __forceinline__ uint32_t color(uint32_t n) const { return meta[n].x; }
     __device__ __forceinline__ uint32_t leafdepth(uint32_t n) const { return meta[n].y; }
@@ -35,6 +42,12 @@ template <bool COUNT, class A>
 __device__ __forceinline__ uint32_t rd_child(const A& a, uint32_t n, uint32_t k, uint64_t& w) {
     if (COUNT) ++w;
     return a.child(n, k);
+}
+// A leaf's rope k (children[k] of the leaf in the .svo words).
+template <bool COUNT, class A>
+__device__ __forceinline__ uint32_t rd_rope(const A& a, uint32_t n, uint32_t k, uint64_t& w) {
+    if (COUNT) ++w;
+    return a.rope(n, k);
 }
 template <bool COUNT, class A>
 __device__ __forceinline__ uint32_t rd_color(const A& a, uint32_t n, uint64_t& w) {
@@ -219,7 +232,7 @@ __device__ __forceinline__ f3 ref_rope(const A& a, uint64_t cap, f3 ro, f3 rd, u
     f3 total = muls(unorm_rgb(rd_color<COUNT>(a, node, w)), step);
     f3 mask;
     uint32_t n = neighbor_index(nbx, nby, nbz, far, mask);
-    node = rd_child<COUNT>(a, node, n, w);
+    node = rd_rope<COUNT>(a, node, n, w);
     offset = add(offset, muls(mul(mask, sgn), side));
     uint64_t it = 0;
     while (node != 0) {
@@ -234,7 +247,7 @@ __device__ __forceinline__ f3 ref_rope(const A& a, uint64_t cap, f3 ro, f3 rd, u
         step = u_max - gmax(u_min, 0.0f);
         total = add(total, muls(unorm_rgb(rd_color<COUNT>(a, node, w)), step));
         n = neighbor_index(nbx, nby, nbz, far, mask);
-        node = rd_child<COUNT>(a, node, n, w);
+        node = rd_rope<COUNT>(a, node, n, w);
         offset = add(offset, muls(mul(mask, sgn), side));
     }
     return total;

@@ -232,7 +232,7 @@ __device__ __forceinline__ f3 svo_pixel(const RenderParams& p, f3 rd, uint32_t* 
     uint32_t* stk_a = lds + wave * 2u * p.lds_slots * kWaveLanes + lane;
     uint32_t* stk_b = stk_a + p.lds_slots * kWaveLanes;
     const FastSvo fs{p.cw, p.meta, p.root_tag, lds_top, p.top_nodes, p.node_count, p.depth_exact, p.table, p.rope_depth,
-                      p.index_mask, p.empty_tag, p.empty_start};
+                      p.index_mask, p.empty_tag, p.empty_start, p.leaf_base};
     if (TRAV == 2) {  // esvo
         float* stk_tmax = reinterpret_cast<float*>(stk_b);
         for (uint32_t s = 0; s < p.stack_slots; ++s) {
@@ -245,7 +245,7 @@ __device__ __forceinline__ f3 svo_pixel(const RenderParams& p, f3 rd, uint32_t* 
         ro = add(ro, smul(gmax(t0, 0.0f), rd));
         if (TAGGED && !COUNT) return fast_esvo(fs, p.cap, p.stack_slots, ro, rd, stk_a,
                                                 stk_tmax);
-        if (TAGGED) return ref_esvo<COUNT>(TaggedNodes{p.cw, p.meta, p.node_count, p.index_mask}, p.cap, p.stack_slots, ro, rd, stk_a, stk_tmax,
+        if (TAGGED) return ref_esvo<COUNT>(TaggedNodes{p.cw, p.meta, p.node_count, p.index_mask, p.leaf_base}, p.cap, p.stack_slots, ro, rd, stk_a, stk_tmax,
                                            words);
         return ref_esvo<COUNT>(RawNodes{p.nodes, p.node_count}, p.cap, p.stack_slots, ro, rd, stk_a, stk_tmax, words);
     }
@@ -253,17 +253,17 @@ __device__ __forceinline__ f3 svo_pixel(const RenderParams& p, f3 rd, uint32_t* 
     if (TRAV == 3) {  // svo-df
         if (TAGGED && !COUNT) return fast_df(fs, p.cap, p.stack_slots, ro, rd, stk_a,
                                              stk_b);
-        if (TAGGED) return ref_df<COUNT>(TaggedNodes{p.cw, p.meta, p.node_count, p.index_mask}, p.cap, p.stack_slots, ro, rd, stk_a, stk_b, words);
+        if (TAGGED) return ref_df<COUNT>(TaggedNodes{p.cw, p.meta, p.node_count, p.index_mask, p.leaf_base}, p.cap, p.stack_slots, ro, rd, stk_a, stk_b, words);
         return ref_df<COUNT>(RawNodes{p.nodes, p.node_count}, p.cap, p.stack_slots, ro, rd, stk_a, stk_b, words);
     }
     if (TRAV == 1) {  // svo-naive
         if (TAGGED && !COUNT) return fast_naive(fs, p.cap, ro, rd);
-        if (TAGGED) return ref_naive<COUNT>(TaggedNodes{p.cw, p.meta, p.node_count, p.index_mask}, p.cap, ro, rd, words);
+        if (TAGGED) return ref_naive<COUNT>(TaggedNodes{p.cw, p.meta, p.node_count, p.index_mask, p.leaf_base}, p.cap, ro, rd, words);
         return ref_naive<COUNT>(RawNodes{p.nodes, p.node_count}, p.cap, ro, rd, words);
     }
     // svo-rope
     if (TAGGED && !COUNT) return fast_rope(fs, p.cap, ro, rd);
-    if (TAGGED) return ref_rope<COUNT>(TaggedNodes{p.cw, p.meta, p.node_count, p.index_mask}, p.cap, ro, rd, words);
+    if (TAGGED) return ref_rope<COUNT>(TaggedNodes{p.cw, p.meta, p.node_count, p.index_mask, p.leaf_base}, p.cap, ro, rd, words);
     return ref_rope<COUNT>(RawNodes{p.nodes, p.node_count}, p.cap, ro, rd, words);
 }
 
