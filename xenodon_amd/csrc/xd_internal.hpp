@@ -68,7 +68,7 @@ struct RenderParams {
     // too large to tag; or the tagged layout: child words + {color, depth}
     const uint32_t* nodes;
     const uint32_t* cw;
-    const uint2* meta;
+    const uint32_t* meta;  // tagged layout: node_count colours, then node_count is_leaf_depth words
     uint32_t root_tag;
     uint32_t node_count;
     uint32_t top_nodes;  // child words of nodes [0, top_nodes) are staged in LDS

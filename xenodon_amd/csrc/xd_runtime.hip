@@ -482,11 +482,24 @@ int xd_volume_from_svo(xd_context* ctx, const void* nodes, uint64_t node_count, 
         }
         to_rope_planes(cw, node_count, v->leaf_base, planes.get());
         cw_buf.reset();
+        // {color, is_leaf_depth} pairs as two planes, so that the colour reads
+        // of neighbouring nodes share cache lines
+        std::unique_ptr<uint32_t[]> meta_planes(new (std::nothrow) uint32_t[node_count * 2]);
+        if (!meta_planes) {
+            xd_volume_free(v);
+            return fail(XD_ERR_OUT_OF_MEMORY, "out of host memory deriving the octree layout");
+        }
+        for (uint64_t i = 0; i < node_count; ++i) {
+            meta_planes[i] = meta[i * 2];
+            meta_planes[node_count + i] = meta[i * 2 + 1];
+        }
+        meta_buf.reset();
         if ((e = hipMalloc(&v->d_cw, node_count * 8 * sizeof(uint32_t))) != hipSuccess ||
-            (e = hipMalloc(&v->d_meta, node_count * sizeof(uint2))) != hipSuccess ||
+            (e = hipMalloc(&v->d_meta, node_count * 2 * sizeof(uint32_t))) != hipSuccess ||
             (e = hipMemcpy(v->d_cw, planes.get(), cw_words * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
                 hipSuccess ||
-            (e = hipMemcpy(v->d_meta, meta, node_count * sizeof(uint2), hipMemcpyHostToDevice)) != hipSuccess) {
+            (e = hipMemcpy(v->d_meta, meta_planes.get(), node_count * 2 * sizeof(uint32_t), hipMemcpyHostToDevice)) !=
+                hipSuccess) {
             xd_volume_free(v);
             return fail_hip(e, "octree upload");
         }

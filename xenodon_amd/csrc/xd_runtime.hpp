@@ -45,7 +45,7 @@ struct xd_volume {
     // has at most 2^31 nodes, else the .svo words as they are
     uint32_t* d_nodes = nullptr;
     uint32_t* d_cw = nullptr;
-    uint2* d_meta = nullptr;
+    uint32_t* d_meta = nullptr;  // colours of all nodes, then their is_leaf_depth words
     uint2* d_table = nullptr;  // start table of the tagged layout (see build_start_table)
     uint32_t rope_depth = 0;   // ropes carry their targets' depth fields (see pack_rope_depths)
     uint32_t index_mask = 0;   // index bits of the tagged child words

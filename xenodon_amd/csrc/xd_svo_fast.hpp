@@ -21,7 +21,7 @@ namespace xd {
 
 struct FastSvo {
     const uint32_t* cw;
-    const uint2* meta;
+    const uint32_t* meta;  // colours of nodes [0, n), then their is_leaf_depth words
     uint32_t root_tag;     // kTagLeaf / kTagEmpty of node 0
     const uint32_t* top;   // LDS copy of the child words of nodes [0, n_top)
     uint32_t n_top;
@@ -60,7 +60,7 @@ __device__ __forceinline__ uint32_t cw_at(const FastSvo& s, uint32_t n, uint32_t
 __device__ __forceinline__ float pmod(float x, float y, float inv_y) { return x - y * __builtin_floorf(x * inv_y); }
 
 __device__ __forceinline__ uint32_t colour_if(const FastSvo& s, uint32_t node, uint32_t tag) {
-    return empty_leaf(s, tag) ? 0u : s.meta[node].x;
+    return empty_leaf(s, tag) ? 0u : s.meta[node];
 }
 
 // The child choices of find() at every level, packed: bit (22 - l) of
@@ -280,7 +280,7 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
                 if (!empty_leaf(s, wd)) {  // an empty leaf adds exactly +0
                     float t_min, t_max;
                     child_trange(corner, side, rrd, bias, ci, t_min, t_max);
-                    total = add(total, muls(unorm_rgb(s.meta[wd & s.index_mask].x), t_max - gmax(t_min, 0.0f)));
+                    total = add(total, muls(unorm_rgb(s.meta[wd & s.index_mask]), t_max - gmax(t_min, 0.0f)));
                 }
                 if (mask != 0) continue;  // next sibling, from registers
             } else if (!(push = s.depth_exact ? mask != 0 : ci != 7u) || sp < cap_sp) {
@@ -306,7 +306,7 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
             const uint32_t e = stk_child[sp * kWaveLanes];
             mask = e & 0xFFu;
             depth = (e >> 11) & 0xFFu;
-            const uint32_t d = s.depth_exact ? depth : (s.meta[node].y & kDepthMask);
+            const uint32_t d = s.depth_exact ? depth : (s.meta[(uint64_t)s.n + node] & kDepthMask);
             const f3 pos = mk(corner.x + side, corner.y + side, corner.z + side);
             side = __builtin_ldexpf(1.0f, -(int)d) * 0.5f;
             // side is a power of two, so mod(pos, 2 side) is exact with the reciprocal
@@ -376,9 +376,8 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
         if (s.rope_depth) {
             d = (int)((wd >> kRopeDepthShift) & 31u);
         } else {
-            const uint2 m = s.meta[node];
-            d = (int)(m.y & kDepthMask);
-            start_col = m.x;
+            d = (int)(s.meta[(uint64_t)s.n + node] & kDepthMask);
+            start_col = s.meta[node];
         }
         float extent = __builtin_ldexpf(1.0f, -d);
         const float inv = __builtin_ldexpf(1.0f, d);
@@ -403,7 +402,7 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
         u_max = min_elem(far);
         step = u_max - gmax(u_min, 0.0f);
         const uint32_t col =
-            empty_leaf(s, tag) ? 0u : (node == start && !s.rope_depth ? start_col : s.meta[node].x);
+            empty_leaf(s, tag) ? 0u : (node == start && !s.rope_depth ? start_col : s.meta[node]);
         n = neighbor_index(nbx, nby, nbz, far, mask);
         wd = rope_at(s, node, n);  // in flight together with the colour
         total = add(total, muls(unorm_rgb(col), step));

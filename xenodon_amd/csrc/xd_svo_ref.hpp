@@ -22,7 +22,7 @@ struct RawNodes {
 
 struct TaggedNodes {
     const uint32_t* child_w;  // tagged child words: 8 per node below leaf_base, then the leaves' ropes as planes
-    const uint2* meta;        // node_count * {color, is_leaf_depth}
+    const uint32_t* meta;     // node_count colours, then node_count is_leaf_depth words
     uint32_t n;
     uint32_t mask;  // index bits of a child word (the volume's index_mask)
     uint32_t leaf_base;
@@ -34,8 +34,8 @@ struct TaggedNodes {
                                           : (uint64_t)l * 8 + k;
         return child_w[i] & mask;
     }
-    __device__ __forceinline__ uint32_t color(uint32_t n) const { return meta[n].x; }
-    __device__ __forceinline__ uint32_t leafdepth(uint32_t n) const { return meta[n].y; }
+    __device__ __forceinline__ uint32_t color(uint32_t i) const { return meta[i]; }
+    __device__ __forceinline__ uint32_t leafdepth(uint32_t i) const { return meta[(uint64_t)n + i]; }
 };
 
 template <bool COUNT, class A>
