@@ -236,7 +236,7 @@ def mixed_views_leg(ctx, scenes, tuning, reps):
             plain_ms += ctx.render(*v)
     b = ctx.batch()
     b.set_tuning(tuning["tile_w"], tuning["tiles_per_grab"], tuning["waves_per_simd"], tuning["class_kernels"],
-                 tuning["interleave"], tuning["mix_classes"])
+                 tuning["interleave"], tuning["mix_classes"], tuning.get("rays_per_lane"))
     b.set_views(views)
     b.render()  # warm
     batch_ms = [b.render()[0] for _ in range(reps)]
@@ -567,7 +567,7 @@ def main():
         t.update(wl.get("tuning", {}))
         t.update(json.loads(args.tuning or "{}"))
         batch.set_tuning(t["tile_w"], t["tiles_per_grab"], t["waves_per_simd"], t["class_kernels"], t["interleave"],
-                         t["mix_classes"])
+                         t["mix_classes"], t.get("rays_per_lane"))
     tuning = batch.autotune(trials=2) if args.autotune else batch.tuning()
     log(f"batched: tuning {tuning}")
     for w in range(max(1, args.warmup // B)):
@@ -596,7 +596,7 @@ def main():
         vol, md = vols[name]
         sub = ctx.batch()
         sub.set_tuning(tuning["tile_w"], tuning["tiles_per_grab"], tuning["waves_per_simd"], tuning["class_kernels"],
-                       tuning["interleave"], tuning["mix_classes"])
+                       tuning["interleave"], tuning["mix_classes"], tuning.get("rays_per_lane"))
         ms_sum = 0.0
         rays_sum = 0
         n_groups = min(len(groups), args.per_traversal_groups)

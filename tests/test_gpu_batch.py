@@ -65,13 +65,17 @@ def test_batch_matches_plain_renders(env):
     _check_against_plain(ctx, views)
 
 
-@pytest.mark.parametrize("tile_w,grab,waves,classes,inter,mix", [(16, 1, 0, 0, 1, 1), (32, 2, 8, 1, 0, 0),
-                                                                 (8, 4, 8, 0, 2, 1), (8, 1, 0, 1, 1, 0),
-                                                                 (16, 2, 0, 1, 2, 1)])
-def test_batch_launch_shapes_do_not_change_pixels(env, tile_w, grab, waves, classes, inter, mix):
+@pytest.mark.parametrize("tile_w,grab,waves,classes,inter,mix,rpl", [(16, 1, 0, 0, 1, 1, 1), (32, 2, 8, 1, 0, 0, 2),
+                                                                     (8, 4, 8, 0, 2, 1, 4), (8, 1, 0, 1, 1, 0, 2),
+                                                                     (16, 2, 0, 1, 2, 1, 1), (8, 1, 8, 0, 2, 2, 2),
+                                                                     (32, 1, 8, 0, 5, 1, 4)])
+def test_batch_launch_shapes_do_not_change_pixels(env, tile_w, grab, waves, classes, inter, mix, rpl):
+    """Every launch shape -- tile shape, grab size, register budget, kernels,
+    tile order, class mixing or XCD affinity, rays per lane (mixed per
+    traversal) -- renders the plain form's pixels."""
     ctx, vols = env
     b = ctx.batch()
-    b.set_tuning([tile_w, 8, tile_w, 32, 16], grab, waves, classes, inter, mix)
+    b.set_tuning([tile_w, 8, tile_w, 32, 16], grab, waves, classes, inter, mix, [rpl, 1, 4, rpl, 2])
     views = _views(ctx, vols, seed=2)
     b.set_views(views)
     b.render()

@@ -76,7 +76,8 @@ def _drain(srv, count):
 @pytest.mark.parametrize("class_kernels", [0, 1])
 def test_served_frames_match_plain(env, class_kernels):
     ctx, data, plain = env
-    srv = ctx.server(tiles_per_launch=1024, tuning={"tile_w": [8, 16, 8, 8, 32], "class_kernels": class_kernels})
+    srv = ctx.server(tiles_per_launch=1024, tuning={"tile_w": [8, 16, 8, 8, 32], "class_kernels": class_kernels,
+                                                    "rays_per_lane": [2, 1, 4, 1, 2] if class_kernels else [1] * 5})
     ids = _load_all(srv, data)
     reqs = _random_requests(40, seed=5, data=data)
     tags = {}

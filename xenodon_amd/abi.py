@@ -70,7 +70,8 @@ class View(ctypes.Structure):
 class BatchTuning(ctypes.Structure):
     _fields_ = [("tile_w", ctypes.c_uint32 * 5), ("tiles_per_grab", ctypes.c_uint32),
                 ("waves_per_simd", ctypes.c_uint32), ("class_kernels", ctypes.c_uint32),
-                ("interleave", ctypes.c_uint32), ("mix_classes", ctypes.c_uint32)]
+                ("interleave", ctypes.c_uint32), ("mix_classes", ctypes.c_uint32),
+                ("rays_per_lane", ctypes.c_uint32 * 5)]
 
 
 class BuildStats(ctypes.Structure):
@@ -296,6 +297,7 @@ class Context:
             c.tuning.class_kernels = tuning.get("class_kernels", 0)
             c.tuning.interleave = tuning.get("interleave", 0)
             c.tuning.mix_classes = tuning.get("mix_classes", 0)
+            c.tuning.rays_per_lane[:] = list(tuning.get("rays_per_lane", [1] * 5))
         h = ctypes.c_void_p()
         check(lib().xd_server_create(self.h, ctypes.byref(c), ctypes.byref(h)))
         return self._adopt(Server(self, h))
@@ -356,8 +358,10 @@ def view_array(views):
     return arr
 
 
-def batch_tuning(tile_w, tiles_per_grab=1, waves_per_simd=8, class_kernels=0, interleave=2, mix_classes=1):
+def batch_tuning(tile_w, tiles_per_grab=1, waves_per_simd=8, class_kernels=0, interleave=2, mix_classes=1,
+                 rays_per_lane=None):
     t = BatchTuning()
+    t.rays_per_lane[:] = list(rays_per_lane or [1] * 5)
     t.tile_w[:] = list(tile_w)
     t.tiles_per_grab = tiles_per_grab
     t.waves_per_simd = waves_per_simd
@@ -379,6 +383,12 @@ def render_static(ctx, views, tuning=None):
     return ms.value, rays.value
 
 
+def tuning_dict(t):
+    return {"tile_w": list(t.tile_w), "tiles_per_grab": t.tiles_per_grab, "waves_per_simd": t.waves_per_simd,
+            "class_kernels": t.class_kernels, "interleave": t.interleave, "mix_classes": t.mix_classes,
+            "rays_per_lane": list(t.rays_per_lane)}
+
+
 class Batch:
     """Persistent ray-tile-queue launch over many views (xd_batch_*)."""
 
@@ -396,19 +406,18 @@ class Batch:
     def tuning(self):
         t = BatchTuning()
         check(lib().xd_batch_get_tuning(self.h, ctypes.byref(t)))
-        return {"tile_w": list(t.tile_w), "tiles_per_grab": t.tiles_per_grab, "waves_per_simd": t.waves_per_simd,
-                "class_kernels": t.class_kernels, "interleave": t.interleave, "mix_classes": t.mix_classes}
+        return tuning_dict(t)
 
-    def set_tuning(self, tile_w, tiles_per_grab=1, waves_per_simd=8, class_kernels=0, interleave=2, mix_classes=1):
+    def set_tuning(self, tile_w, tiles_per_grab=1, waves_per_simd=8, class_kernels=0, interleave=2, mix_classes=1,
+                   rays_per_lane=None):
         """Defaults are the library's starting launch shape."""
-        t = batch_tuning(tile_w, tiles_per_grab, waves_per_simd, class_kernels, interleave, mix_classes)
+        t = batch_tuning(tile_w, tiles_per_grab, waves_per_simd, class_kernels, interleave, mix_classes, rays_per_lane)
         check(lib().xd_batch_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, trials=3):
         t = BatchTuning()
         check(lib().xd_batch_autotune(self.h, trials, ctypes.byref(t)))
-        return {"tile_w": list(t.tile_w), "tiles_per_grab": t.tiles_per_grab, "waves_per_simd": t.waves_per_simd,
-                "class_kernels": t.class_kernels, "interleave": t.interleave, "mix_classes": t.mix_classes}
+        return tuning_dict(t)
 
     def render(self):
         """Returns (ms, rays)."""

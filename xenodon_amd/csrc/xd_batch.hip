@@ -41,6 +41,8 @@ __global__ void k_batch_reset(uint32_t* counters) {
 // budget is its own, not by the all-class kernel).
 template <int TRAV, bool PIPE = (TRAV == XD_DDA)>
 __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds);
+template <int TRAV, bool PIPE>
+__device__ __forceinline__ void shade_pixel(const BatchView& V, uint32_t ix, uint32_t iy, uint32_t* lds);
 
 // Any traversal: the kernel for all classes at once dispatches on the view.
 template <>
@@ -60,9 +62,18 @@ __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, ui
     const uint32_t sh = V.tw_shift;
     const uint32_t tx = tile % V.tiles_x, ty = tile / V.tiles_x;
     const uint32_t ix = (tx << sh) + (lane & ((1u << sh) - 1u));
-    const uint32_t iy = ty * (64u >> sh) + (lane >> sh);
     const RenderParams& p = V.p;
-    if (ix >= p.view.out_ext[0] || iy >= p.view.out_ext[1]) return;
+    // rays per lane: the tile is rpl rows of wave tiles, traced one row after
+    // another by the same lanes
+    for (uint32_t r = 0; r < V.rpl; ++r) {
+        const uint32_t iy = (ty * V.rpl + r) * (64u >> sh) + (lane >> sh);
+        if (ix < p.view.out_ext[0] && iy < p.view.out_ext[1]) shade_pixel<TRAV, PIPE>(V, ix, iy, lds);
+    }
+}
+
+template <int TRAV, bool PIPE>
+__device__ __forceinline__ void shade_pixel(const BatchView& V, uint32_t ix, uint32_t iy, uint32_t* lds) {
+    const RenderParams& p = V.p;
     const f3 rd = ray_dir(p.cam, p.view, ix, iy);
     uint64_t words = 0;
     f3 c;
@@ -147,7 +158,7 @@ const BatchKernel kKernels[2][kClasses + 1] = {
 uint32_t tw_shift_of(uint32_t tw) { return tw == 32 ? 5u : (tw == 16 ? 4u : 3u); }
 
 void make_batch_view(const xd_volume* vol, int traversal, const xd_camera* cam, const xd_uniforms* u, uint32_t* out,
-                     uint32_t pitch, uint32_t tile_w, BatchView& B, uint64_t* ntiles) {
+                     uint32_t pitch, uint32_t tile_w, uint32_t rays_per_lane, BatchView& B, uint64_t* ntiles) {
     memset(&B, 0, sizeof(B));
     fill_params(B.p, vol, traversal, cam, u);
     B.p.top_nodes = 0;  // no LDS top cache: views of one launch may use different volumes
@@ -156,7 +167,8 @@ void make_batch_view(const xd_volume* vol, int traversal, const xd_camera* cam, 
     B.traversal = (uint32_t)traversal;
     if (tile_w != 8 && tile_w != 16 && tile_w != 32) tile_w = 8;
     B.tw_shift = tw_shift_of(tile_w);
-    const uint32_t th = 64u / tile_w;
+    B.rpl = rays_per_lane == 2 || rays_per_lane == 4 ? rays_per_lane : 1u;
+    const uint32_t th = 64u / tile_w * B.rpl;
     B.tiles_x = (u->output_region.extent[0] + tile_w - 1) / tile_w;
     *ntiles = (uint64_t)B.tiles_x * ((u->output_region.extent[1] + th - 1) / th);
 }
@@ -327,7 +339,7 @@ int build_tables(const xd_view* views, uint32_t n, const xd_batch_tuning& tuning
         BatchView& B = bv[i];
         uint64_t nt = 0;
         make_batch_view(v.volume, v.traversal, &v.camera, &v.uniforms, v.frame->d_pixels, v.frame->w,
-                        tuning.tile_w[v.traversal], B, &nt);
+                        tuning.tile_w[v.traversal], tuning.rays_per_lane[v.traversal], B, &nt);
         const uint32_t W = v.uniforms.output_region.extent[0], H = v.uniforms.output_region.extent[1];
         const uint32_t tiles_y = (uint32_t)(nt / B.tiles_x);
         if (nt > kMaxTilesPerView)
@@ -340,7 +352,7 @@ int build_tables(const xd_view* views, uint32_t n, const xd_batch_tuning& tuning
         if (order == 1 || order == 5) {
             // views of one volume, traversal, size and tile shape form a group
             // whose tiles are dealt together below
-            const uint64_t key[6] = {(uint64_t)(uintptr_t)v.volume, (uint64_t)c, W, H, B.tw_shift, 0};
+            const uint64_t key[6] = {(uint64_t)(uintptr_t)v.volume, (uint64_t)c, W, H, B.tw_shift, B.rpl};
             size_t g = 0;
             while (g < groups.size() && memcmp(groups[g].key, key, sizeof key) != 0) ++g;
             if (g == groups.size()) groups.push_back({{key[0], key[1], key[2], key[3], key[4], key[5]}, {}, B.tiles_x, tiles_y});
@@ -505,7 +517,10 @@ int check_views(const xd_context* ctx, const xd_view* views, uint32_t count) {
 // 8 waves per SIMD, tiles in squares of 8.
 xd_batch_tuning default_tuning() {
     xd_batch_tuning t{};
-    for (int i = 0; i < 5; ++i) t.tile_w[i] = 8;
+    for (int i = 0; i < 5; ++i) {
+        t.tile_w[i] = 8;
+        t.rays_per_lane[i] = 1;
+    }
     t.tiles_per_grab = 1;
     t.interleave = 2;
     t.waves_per_simd = 8;
@@ -524,6 +539,9 @@ int check_tuning(const xd_batch_tuning* t) {
     if (t->class_kernels > 1) return fail(XD_ERR_INVALID_ARGUMENT, "class_kernels must be 0 or 1");
     if (t->interleave > 5) return fail(XD_ERR_INVALID_ARGUMENT, "interleave (tile order) must be 0..5");
     if (t->mix_classes > 2) return fail(XD_ERR_INVALID_ARGUMENT, "mix_classes must be 0, 1 or 2");
+    for (int i = 0; i < 5; ++i)
+        if (t->rays_per_lane[i] > 4 || t->rays_per_lane[i] == 3)
+            return fail(XD_ERR_INVALID_ARGUMENT, "rays_per_lane %u is not 1, 2 or 4 (0 = 1)", t->rays_per_lane[i]);
     return XD_OK;
 }
 
@@ -654,6 +672,7 @@ int xd_batch_autotune(xd_batch* b, uint32_t trials, xd_batch_tuning* chosen) {
         if (sub.empty()) continue;
         b->views = sub;
         rc = sweep(b, trials, &b->tuning.tile_w[t], {8u, 16u, 32u});
+        if (!rc) rc = sweep(b, trials, &b->tuning.rays_per_lane[t], {1u, 2u, 4u});
     }
     b->keep_mixed = false;
     // then the launch shape, on the whole batch, one field at a time from

@@ -15,7 +15,7 @@ struct BatchView {
     uint32_t traversal;
     uint32_t tw_shift;  // log2 of the tile width
     uint32_t tiles_x;
-    uint32_t pad;
+    uint32_t rpl;       // rays per lane: a tile is 64 * rpl pixels, rpl rows of wave tiles
 };
 
 constexpr uint32_t kQueues = 8;                 // one tile range per XCD, per traversal class
@@ -29,7 +29,7 @@ uint32_t tw_shift_of(uint32_t tile_w);
 // returns its number of wave tiles in *ntiles.  The LDS stack stride
 // B.p.lds_slots is left at the view's own depth; the launch sets it.
 void make_batch_view(const xd_volume* vol, int traversal, const xd_camera* cam, const xd_uniforms* u, uint32_t* out,
-                     uint32_t pitch, uint32_t tile_w, BatchView& B, uint64_t* ntiles);
+                     uint32_t pitch, uint32_t tile_w, uint32_t rays_per_lane, BatchView& B, uint64_t* ntiles);
 
 // Stack slots a view needs in LDS (0 for traversals without a stack).
 uint32_t batch_view_slots(const BatchView& B);

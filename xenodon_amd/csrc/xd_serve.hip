@@ -151,6 +151,7 @@ struct xd_server {
     size_t views_off = 0, qbeg_off = 0, counters_off = 0, tiles_off = 0, table_bytes = 0;
     std::map<uint32_t, uint32_t> grid_of;  // (traversal << 16 | stack slots) -> persistent grid
     uint32_t tile_w[5] = {8, 8, 8, 8, 8};
+    uint32_t rpl[5] = {1, 1, 1, 1, 1};
     uint32_t grab = 1;
     bool w8 = false;
     bool class_kernels = false;
@@ -539,13 +540,19 @@ int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** o
     if (c.tuning.waves_per_simd != 0 && c.tuning.waves_per_simd != 8)
         return fail(XD_ERR_INVALID_ARGUMENT, "waves_per_simd must be 0 or 8");
     if (c.tuning.class_kernels > 1) return fail(XD_ERR_INVALID_ARGUMENT, "class_kernels must be 0 or 1");
+    for (int t = 0; t < 5; ++t)
+        if (c.tuning.rays_per_lane[t] > 4 || c.tuning.rays_per_lane[t] == 3)
+            return fail(XD_ERR_INVALID_ARGUMENT, "rays_per_lane %u is not 1, 2 or 4 (0 = 1)", c.tuning.rays_per_lane[t]);
     if (!(c.latency_target_ms >= 0)) return fail(XD_ERR_INVALID_ARGUMENT, "latency_target_ms must be >= 0");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
     auto* s = new xd_server();
     s->ctx = ctx;
     s->t0 = std::chrono::steady_clock::now();
-    for (int t = 0; t < 5; ++t) s->tile_w[t] = c.tuning.tile_w[t];
+    for (int t = 0; t < 5; ++t) {
+        s->tile_w[t] = c.tuning.tile_w[t];
+        s->rpl[t] = c.tuning.rays_per_lane[t] ? c.tuning.rays_per_lane[t] : 1u;
+    }
     s->grab = c.tuning.tiles_per_grab;
     s->w8 = c.tuning.waves_per_simd == 8;
     s->class_kernels = c.tuning.class_kernels != 0;
@@ -664,7 +671,8 @@ int xd_server_submit(xd_server* s, const xd_request* req, uint64_t* request_id) 
         u.model_dim[i] = (uint32_t)(vol->kind == XD_VOLUME_GRID ? vol->dims[i] : vol->side);
     }
     u.emission_coeff = r.emission_coeff;
-    make_batch_view(vol, r.traversal, &r.camera, &u, nullptr, r.width, s->tile_w[r.traversal], q->bv, &q->tiles_total);
+    make_batch_view(vol, r.traversal, &r.camera, &u, nullptr, r.width, s->tile_w[r.traversal], s->rpl[r.traversal], q->bv,
+                    &q->tiles_total);
     if (q->tiles_total > kMaxTilesPerView) {
         delete q;
         return fail(XD_ERR_INVALID_ARGUMENT, "request has %llu tiles (limit %u)", (unsigned long long)q->tiles_total,
