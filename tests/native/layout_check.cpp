@@ -12,7 +12,7 @@
 //   * the start table: for `points` seeded points, the node find() reaches
 //     from the root over the .svo words (svo_naive.comp:8-27 in FP32) equals
 //     the node reached by resuming the walk over the tagged words from the
-//     table entry of the point's level-6 cell, with the same offset and side;
+//     table entry of the point's level-kTableLevel cell, with the same offset and side;
 //   * rope depths (when packed): bits 25..29 of every leaf child word hold the
 //     depth field of the node it names;
 //   * the device form (to_rope_planes): the words of the nodes before the first
@@ -231,7 +231,8 @@ int main(int argc, char** argv) {
     };
     for (uint64_t q = 0; q < points; ++q) {
         float pos[3] = {next_float(), next_float(), next_float()};
-        if (q % 7 == 0) pos[q % 3] = std::ldexp((float)(s & 63u), -6);  // on a level-6 cell boundary
+        if (q % 7 == 0)  // on a table cell boundary
+            pos[q % 3] = std::ldexp((float)(s & ((1u << kTableLevel) - 1u)), -(int)kTableLevel);
         float o1[3], o2[3], s1, s2;
         uint32_t lvl;
         const uint32_t a = find_svo(w.data(), pos, o1, &s1, &lvl);

@@ -73,8 +73,8 @@ def test_volume_footprint_without_device():
     # 128^3: 32^3 bricks, 16^3 macro cells of 8 voxels (128 words)
     assert abi.volume_footprint(abi.VOLUME_GRID, (128, 128, 128)) == 32 ** 3 * 257 + 128 * 4
     # octree: 32 B of child words + 8 B of {color, depth} per node, plus the
-    # 64^3-cell start table of 8 B entries
-    assert abi.volume_footprint(abi.VOLUME_SVO, node_count=1000) == 40 * 1000 + 8 * 64 ** 3
+    # 256^3-cell start table of 8 B entries
+    assert abi.volume_footprint(abi.VOLUME_SVO, node_count=1000) == 40 * 1000 + 8 * 256 ** 3
     for bad in [dict(kind=abi.VOLUME_GRID, dims=(0, 4, 4)), dict(kind=abi.VOLUME_SVO, node_count=0),
                 dict(kind=7, dims=(4, 4, 4))]:
         try:

@@ -45,8 +45,8 @@ def test_c5_volumes_resident(c5):
     assert gv.info().device_bytes >= 32 << 30
     assert st["side"] == 2048 and st["depth"] == 11
     assert 1 << 30 < nodes.shape[0] < 1 << 31
-    # 40 B per node either way; the tagged layout adds the 2 MiB start table
-    assert sv.info().device_bytes == nodes.nbytes + 8 * 64 ** 3
+    # 40 B per node either way; the tagged layout adds the 128 MiB start table
+    assert sv.info().device_bytes == nodes.nbytes + 8 * 256 ** 3
     print(f"2048^3 rope octree: {nodes.shape[0]} nodes, {nodes.nbytes / 1e9:.1f} GB")
 
 

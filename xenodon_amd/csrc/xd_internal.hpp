@@ -36,9 +36,12 @@ constexpr uint32_t kRopeIndexMask = (1u << kRopeDepthShift) - 1u;
 // at most 1 + 8 + 64 + 512 = 585 nodes (18.7 KB of child words)
 constexpr uint32_t kTopLevels = 4;
 // Start table of a tagged octree: for each cell of the level-kTableLevel grid
-// (64^3 cells) the node that find() reaches at that level, or the leaf that
-// ends the walk above it, with its level.  2 MiB per octree volume.
-constexpr uint32_t kTableLevel = 6;
+// (256^3 cells) the node that find() reaches at that level, or the leaf that
+// ends the walk above it, with its level.  128 MiB per octree volume.  On the
+// 512^3 octree (depth 9) svo-naive ran at 743 / 807 / 880 / 906 Mrays/s plain
+// and 1120 / 1286 / 1433 / 1393 batched with tables of levels 6 / 7 / 8 / 9
+// (2 MiB / 16 MiB / 128 MiB / 1 GiB).
+constexpr uint32_t kTableLevel = 8;
 constexpr uint64_t kTableCells = 1ull << (3 * kTableLevel);
 constexpr uint64_t kTableBytes = kTableCells * 8;
 

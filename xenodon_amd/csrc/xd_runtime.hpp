@@ -103,11 +103,12 @@ bool octree_is_tree(const uint32_t* w, uint64_t n);
 // themselves instead of reading the field back.
 bool octree_depths_exact(const uint32_t* w, uint64_t n, bool tree);
 // Start table of a tagged layout (child_w as build_tagged_layout writes it,
-// root_tag the root's own tags): entry (z << 12 | y << 6 | x) of the level-6
-// grid = {tagged word of the node find() reaches at level 6 for any point of
-// cell (x, y, z), or of the leaf that ends the walk above it; that node's
-// level}.  The root's word is 0 | root_tag.  The walk follows child words
-// for at most 6 levels, so any tree, cyclic ones included, gives a table.
+// root_tag the root's own tags): entry (z << 2L | y << L | x) of the level-L
+// grid (L = kTableLevel) = {tagged word of the node find() reaches at level L
+// for any point of cell (x, y, z), or of the leaf that ends the walk above it;
+// that node's level}.  The root's word is 0 | root_tag.  The walk follows
+// child words for at most L levels, so any tree, cyclic ones included, gives
+// a table.
 void build_start_table(const uint32_t* child_w, uint32_t root_tag, uint32_t index_mask, uint2* table);
 // When the tree has fewer than 2^25 nodes and every depth field is below 32,
 // ORs into every child word of every leaf (its ropes) the depth field of the

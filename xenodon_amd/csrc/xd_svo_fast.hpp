@@ -79,7 +79,7 @@ __device__ __forceinline__ uint32_t level_bits(float p) {
 
 // svo_naive.comp:8-27 with tagged pointers.  `tag` returns the found node's
 // tags.  With a start table the walk begins at the node the table holds for
-// the point's level-6 cell (found from the point's level bits): the same node
+// the point's level-kTableLevel cell (found from the point's level bits): the same node
 // the first six levels of the walk reach, with the same offset and extent,
 // which are exact dyadic values.
 __device__ __forceinline__ uint32_t fast_find(const FastSvo& s, f3 pos, f3& base, float& side, uint32_t& tag) {
