@@ -224,11 +224,12 @@ typedef struct {
                              traversals in proportion to their work, so that
                              each XCD's L2 serves one or two traversals' data
                              (batches only; the served form reads 2 as 1) */
-    uint32_t rays_per_lane[5]; /* per traversal, rays each lane traces per tile (1, 2
-                                  or 4; 0 = 1): a tile is rays_per_lane rows of
-                                  wave tiles (64 rays each), traced one row after
-                                  another by the same wavefront, so a wavefront
-                                  takes 64 * rays_per_lane rays per grab */
+    uint32_t rays_per_lane[5]; /* per traversal, rays each lane traces per tile column
+                                  (1, 2 or 4; 0 = 1): that many rows of wave
+                                  tiles are queued together and taken in one
+                                  grab, then traced in turn by the same
+                                  wavefront (64 * rays_per_lane rays per grab).
+                                  The served form reads it as 1 */
 } xd_batch_tuning;
 
 typedef struct xd_batch xd_batch;

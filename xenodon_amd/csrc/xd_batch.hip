@@ -41,8 +41,6 @@ __global__ void k_batch_reset(uint32_t* counters) {
 // budget is its own, not by the all-class kernel).
 template <int TRAV, bool PIPE = (TRAV == XD_DDA)>
 __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds);
-template <int TRAV, bool PIPE>
-__device__ __forceinline__ void shade_pixel(const BatchView& V, uint32_t ix, uint32_t iy, uint32_t* lds);
 
 // Any traversal: the kernel for all classes at once dispatches on the view.
 template <>
@@ -62,18 +60,9 @@ __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, ui
     const uint32_t sh = V.tw_shift;
     const uint32_t tx = tile % V.tiles_x, ty = tile / V.tiles_x;
     const uint32_t ix = (tx << sh) + (lane & ((1u << sh) - 1u));
+    const uint32_t iy = ty * (64u >> sh) + (lane >> sh);
     const RenderParams& p = V.p;
-    // rays per lane: the tile is rpl rows of wave tiles, traced one row after
-    // another by the same lanes
-    for (uint32_t r = 0; r < V.rpl; ++r) {
-        const uint32_t iy = (ty * V.rpl + r) * (64u >> sh) + (lane >> sh);
-        if (ix < p.view.out_ext[0] && iy < p.view.out_ext[1]) shade_pixel<TRAV, PIPE>(V, ix, iy, lds);
-    }
-}
-
-template <int TRAV, bool PIPE>
-__device__ __forceinline__ void shade_pixel(const BatchView& V, uint32_t ix, uint32_t iy, uint32_t* lds) {
-    const RenderParams& p = V.p;
+    if (ix >= p.view.out_ext[0] || iy >= p.view.out_ext[1]) return;
     const f3 rd = ray_dir(p.cam, p.view, ix, iy);
     uint64_t words = 0;
     f3 c;
@@ -95,11 +84,12 @@ __device__ __forceinline__ void shade_pixel(const BatchView& V, uint32_t ix, uin
 // The tiles of one traversal class: kQueues ranges (one per XCD) of
 // qbeg/counters.  A wave takes tiles from its own XCD's range with one atomic
 // per grab and moves on to the other ranges when that one is drained; every
-// wave leaves once all ranges are drained.
+// wave leaves once all ranges are drained.  grabs packs the tiles per grab of
+// each class, 6 bits each (class c at bit 6c).
 template <int TRAV>
 __device__ __forceinline__ void class_body(const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles,
                                            const uint32_t* __restrict__ qbeg, uint32_t* __restrict__ counters,
-                                           uint32_t per_grab, uint32_t mix, uint32_t* lds) {
+                                           uint32_t grabs, uint32_t mix, uint32_t* lds) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t home = blockIdx.x % kQueues;
     constexpr uint32_t kRanges = TRAV == (int)kClasses ? kClasses * kQueues : kQueues;
@@ -112,6 +102,7 @@ __device__ __forceinline__ void class_body(const BatchView* __restrict__ views, 
                                 ? ((qi + first) % kClasses) * kQueues + (home + qi / kClasses) % kQueues
                                 : (home + qi) % kQueues;
         const uint32_t beg = qbeg[qq], end = qbeg[qq + 1];
+        const uint32_t per_grab = (grabs >> (6u * (TRAV == (int)kClasses ? qq / kQueues : (uint32_t)TRAV))) & 63u;
         for (;;) {
             uint32_t t = 0;
             if (lane == 0) t = atomicAdd(&counters[qq], per_grab);
@@ -134,18 +125,18 @@ __device__ __forceinline__ void class_body(const BatchView* __restrict__ views, 
 template <int TRAV>
 __global__ __launch_bounds__(256) void k_batch(const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles,
                                                const uint32_t* __restrict__ qbeg, uint32_t* __restrict__ counters,
-                                               uint32_t per_grab, uint32_t mix) {
+                                               uint32_t grabs, uint32_t mix) {
     extern __shared__ uint32_t lds[];
-    class_body<TRAV>(views, tiles, qbeg, counters, per_grab, mix, lds);
+    class_body<TRAV>(views, tiles, qbeg, counters, grabs, mix, lds);
 }
 
 // ... or capped for 8 waves per SIMD (registers beyond the cap spill).
 template <int TRAV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_w8(
     const BatchView* __restrict__ views, const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ qbeg,
-    uint32_t* __restrict__ counters, uint32_t per_grab, uint32_t mix) {
+    uint32_t* __restrict__ counters, uint32_t grabs, uint32_t mix) {
     extern __shared__ uint32_t lds[];
-    class_body<TRAV>(views, tiles, qbeg, counters, per_grab, mix, lds);
+    class_body<TRAV>(views, tiles, qbeg, counters, grabs, mix, lds);
 }
 
 namespace {
@@ -156,6 +147,16 @@ const BatchKernel kKernels[2][kClasses + 1] = {
 }  // namespace
 
 uint32_t tw_shift_of(uint32_t tw) { return tw == 32 ? 5u : (tw == 16 ? 4u : 3u); }
+
+uint32_t pack_grabs(uint32_t tiles_per_grab, const uint32_t rays_per_lane[kClasses]) {
+    const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(tiles_per_grab, 8));
+    uint32_t packed = 0;
+    for (uint32_t c = 0; c < kClasses; ++c) {
+        const uint32_t r = rays_per_lane && (rays_per_lane[c] == 2 || rays_per_lane[c] == 4) ? rays_per_lane[c] : 1u;
+        packed |= (g * r) << (6 * c);
+    }
+    return packed;
+}
 
 void make_batch_view(const xd_volume* vol, int traversal, const xd_camera* cam, const xd_uniforms* u, uint32_t* out,
                      uint32_t pitch, uint32_t tile_w, uint32_t rays_per_lane, BatchView& B, uint64_t* ntiles) {
@@ -168,7 +169,7 @@ void make_batch_view(const xd_volume* vol, int traversal, const xd_camera* cam, 
     if (tile_w != 8 && tile_w != 16 && tile_w != 32) tile_w = 8;
     B.tw_shift = tw_shift_of(tile_w);
     B.rpl = rays_per_lane == 2 || rays_per_lane == 4 ? rays_per_lane : 1u;
-    const uint32_t th = 64u / tile_w * B.rpl;
+    const uint32_t th = 64u / tile_w;
     B.tiles_x = (u->output_region.extent[0] + tile_w - 1) / tile_w;
     *ntiles = (uint64_t)B.tiles_x * ((u->output_region.extent[1] + th - 1) / th);
 }
@@ -195,14 +196,13 @@ int batch_grid(int device, int traversal, uint32_t slots, bool w8, uint32_t* gri
 }
 
 hipError_t launch_batch(const BatchView* views, const uint32_t* tiles, const uint32_t* qbeg, uint32_t* counters,
-                        uint32_t grab, const BatchClasses& cls, bool w8, bool mix, hipStream_t s) {
-    grab = std::max<uint32_t>(1, std::min<uint32_t>(grab, 8));
+                        uint32_t grabs, const BatchClasses& cls, bool w8, bool mix, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, counters);
     for (int c = 0; c <= (int)kClasses; ++c) {
         if (!cls.grid[c]) continue;  // no tiles for this kernel
         const uint32_t base = c < (int)kClasses ? c * kQueues : 0;
         hipLaunchKernelGGL(kKernels[w8][c], dim3(cls.grid[c]), dim3(256), batch_lds_bytes(cls.slots[c]), s, views,
-                           tiles, qbeg + base, counters + base, grab, mix ? 1u : 0u);
+                           tiles, qbeg + base, counters + base, grabs, mix ? 1u : 0u);
     }
     return hipGetLastError();
 }
@@ -329,7 +329,7 @@ int build_tables(const xd_view* views, uint32_t n, const xd_batch_tuning& tuning
     struct Group {
         uint64_t key[6];
         std::vector<uint32_t> views;
-        uint32_t tiles_x, tiles_y;
+        uint32_t tiles_x, tiles_y;  // tiles_y: groups of rays_per_lane rows
     };
     std::vector<Group> groups;
     BatchClasses cls{};
@@ -342,9 +342,14 @@ int build_tables(const xd_view* views, uint32_t n, const xd_batch_tuning& tuning
                         tuning.tile_w[v.traversal], tuning.rays_per_lane[v.traversal], B, &nt);
         const uint32_t W = v.uniforms.output_region.extent[0], H = v.uniforms.output_region.extent[1];
         const uint32_t tiles_y = (uint32_t)(nt / B.tiles_x);
-        if (nt > kMaxTilesPerView)
-            return fail(XD_ERR_INVALID_ARGUMENT, "view %u has %llu tiles (limit %u)", i, (unsigned long long)nt,
-                        kMaxTilesPerView);
+        // rays per lane: the rows of a wave tile column are taken rpl at a
+        // time (one grab of rpl entries, traced in turn by the same lanes);
+        // rows past the bottom of a view are queued as no-op entries so that
+        // every grab stays aligned to a group of rpl rows
+        const uint32_t rpl = B.rpl, big_y = (tiles_y + rpl - 1) / rpl;
+        if ((uint64_t)big_y * rpl * B.tiles_x > kMaxTilesPerView)
+            return fail(XD_ERR_INVALID_ARGUMENT, "view %u has %llu tiles (limit %u)", i,
+                        (unsigned long long)big_y * rpl * B.tiles_x, kMaxTilesPerView);
         const uint32_t c = (uint32_t)v.traversal;
         cls.slots[c] = std::max(cls.slots[c], batch_view_slots(B));
         cls.grid[c] = 1;
@@ -355,19 +360,20 @@ int build_tables(const xd_view* views, uint32_t n, const xd_batch_tuning& tuning
             const uint64_t key[6] = {(uint64_t)(uintptr_t)v.volume, (uint64_t)c, W, H, B.tw_shift, B.rpl};
             size_t g = 0;
             while (g < groups.size() && memcmp(groups[g].key, key, sizeof key) != 0) ++g;
-            if (g == groups.size()) groups.push_back({{key[0], key[1], key[2], key[3], key[4], key[5]}, {}, B.tiles_x, tiles_y});
+            if (g == groups.size()) groups.push_back({{key[0], key[1], key[2], key[3], key[4], key[5]}, {}, B.tiles_x, big_y});
             groups[g].views.push_back(i);
         } else {
-            // squares of sq x sq tiles (sq = 1: plain rows), row-major within
-            // each band's squares
+            // squares of sq x sq groups of rows (sq = 1: plain rows),
+            // row-major within each band's squares
             const uint32_t sq = order == 2 ? 8u : order == 3 ? 4u : order == 4 ? 16u : 1u;
-            for (uint32_t ty0 = 0; ty0 < tiles_y; ty0 += sq) {
-                const uint32_t band = (uint32_t)((uint64_t)ty0 * kQueues / tiles_y);
-                const uint32_t ty1 = std::min(tiles_y, ty0 + sq);
+            for (uint32_t ty0 = 0; ty0 < big_y; ty0 += sq) {
+                const uint32_t band = (uint32_t)((uint64_t)ty0 * kQueues / big_y);
+                const uint32_t ty1 = std::min(big_y, ty0 + sq);
                 for (uint32_t tx0 = 0; tx0 < B.tiles_x; tx0 += sq)
                     for (uint32_t ty = ty0; ty < ty1; ++ty)
                         for (uint32_t tx = tx0; tx < std::min(B.tiles_x, tx0 + sq); ++tx)
-                            q[c * kQueues + band].push_back((i << 20) | (ty * B.tiles_x + tx));
+                            for (uint32_t r = 0; r < rpl; ++r)
+                                q[c * kQueues + band].push_back((i << 20) | ((ty * rpl + r) * B.tiles_x + tx));
             }
         }
         T.total_rays += (uint64_t)W * H;
@@ -382,7 +388,9 @@ int build_tables(const xd_view* views, uint32_t n, const xd_batch_tuning& tuning
                 for (uint32_t ty = ty0; ty < std::min(G.tiles_y, ty0 + gsq); ++ty)
                     for (uint32_t tx = tx0; tx < std::min(G.tiles_x, tx0 + gsq); ++tx)
                         for (uint32_t i : G.views)
-                            q[(uint32_t)G.key[1] * kQueues + band].push_back((i << 20) | (ty * G.tiles_x + tx));
+                            for (uint32_t r = 0; r < (uint32_t)G.key[5]; ++r)
+                                q[(uint32_t)G.key[1] * kQueues + band].push_back(
+                                    (i << 20) | ((ty * (uint32_t)G.key[5] + r) * G.tiles_x + tx));
         }
     // A batch of one traversal runs that traversal's own kernel whatever the
     // tuning says: with one class there is nothing to mix, and the class
@@ -451,7 +459,7 @@ int capture(xd_batch* b, const BatchClasses& need) {
         if (rc) return rc;
     }
     hipStream_t s = b->ctx->stream;
-    const uint32_t grab = std::max<uint32_t>(1, std::min<uint32_t>(b->tuning.tiles_per_grab, 8));
+    const uint32_t grab = pack_grabs(b->tuning.tiles_per_grab, b->tuning.rays_per_lane);
     if ((e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return fail_hip(e, "hipStreamBeginCapture");
     const hipError_t le =
@@ -471,7 +479,7 @@ int prepare(xd_batch* b) {
     BatchClasses need{};
     int rc = upload_tables(b, &need);
     if (rc) return rc;
-    const uint32_t grab = std::max<uint32_t>(1, std::min<uint32_t>(b->tuning.tiles_per_grab, 8));
+    const uint32_t grab = pack_grabs(b->tuning.tiles_per_grab, b->tuning.rays_per_lane);
     bool stale = !b->exec || grab != b->graph_grab || b->graph_w8 != (uint32_t)(b->tuning.waves_per_simd == 8) ||
                  b->graph_mix != b->tuning.mix_classes;
     for (int c = 0; c <= (int)kClasses; ++c)
@@ -783,7 +791,8 @@ int xd_batch_render_static(xd_context* ctx, const xd_view* views, uint32_t count
         (total && (e = hipMemcpyAsync(d_tiles, T.tiles.data(), sizeof(uint32_t) * total, hipMemcpyHostToDevice, s)) != hipSuccess) ||
         (e = hipMemcpyAsync(d_qbeg, T.qbeg, sizeof(T.qbeg), hipMemcpyHostToDevice, s)) != hipSuccess ||
         (e = hipEventRecord(ctx->ev0, s)) != hipSuccess ||
-        (e = launch_batch(d_views, d_tiles, d_qbeg, d_counters, tun.tiles_per_grab, cls, w8, tun.mix_classes != 0, s)) !=
+        (e = launch_batch(d_views, d_tiles, d_qbeg, d_counters, pack_grabs(tun.tiles_per_grab, tun.rays_per_lane), cls,
+                          w8, tun.mix_classes != 0, s)) !=
             hipSuccess ||
         (e = hipEventRecord(ctx->ev1, s)) != hipSuccess || (e = hipEventSynchronize(ctx->ev1)) != hipSuccess) {
         (void)hipStreamSynchronize(s);  // no copy may still read the host tables when they go

@@ -15,7 +15,7 @@ struct BatchView {
     uint32_t traversal;
     uint32_t tw_shift;  // log2 of the tile width
     uint32_t tiles_x;
-    uint32_t rpl;       // rays per lane: a tile is 64 * rpl pixels, rpl rows of wave tiles
+    uint32_t rpl;       // rays per lane: rows of wave tiles queued and grabbed together
 };
 
 constexpr uint32_t kQueues = 8;                 // one tile range per XCD, per traversal class
@@ -24,6 +24,11 @@ constexpr uint32_t kMaxBatchViews = 1u << 12;   // a tile entry is (view << 20 |
 constexpr uint32_t kMaxTilesPerView = 1u << 20;
 
 uint32_t tw_shift_of(uint32_t tile_w);
+
+// The per-class tiles per grab the kernels take, packed 6 bits per class:
+// tiles_per_grab x rays_per_lane[c] (null: 1), so that a grab holds whole
+// groups of rays_per_lane rows.
+uint32_t pack_grabs(uint32_t tiles_per_grab, const uint32_t rays_per_lane[kClasses]);
 
 // Fills B for one output (frame pixels at out with row pitch `pitch`) and
 // returns its number of wave tiles in *ntiles.  The LDS stack stride
@@ -54,8 +59,9 @@ struct BatchClasses {
 // Enqueues the counter reset and the present kernels of cls on s.  qbeg holds
 // kClasses * kQueues + 1 range starts (class c's ranges are entries
 // c*kQueues .. c*kQueues+kQueues); counters holds kClasses * kQueues.
-// mix: the all-class kernel's waves start on different traversal classes.
+// grabs: tiles per grab of each class (pack_grabs).  mix: the all-class
+// kernel's waves start on different traversal classes.
 hipError_t launch_batch(const BatchView* views, const uint32_t* tiles, const uint32_t* qbeg, uint32_t* counters,
-                        uint32_t grab, const BatchClasses& cls, bool w8, bool mix, hipStream_t s);
+                        uint32_t grabs, const BatchClasses& cls, bool w8, bool mix, hipStream_t s);
 
 }  // namespace xd

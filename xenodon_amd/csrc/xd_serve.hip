@@ -151,7 +151,6 @@ struct xd_server {
     size_t views_off = 0, qbeg_off = 0, counters_off = 0, tiles_off = 0, table_bytes = 0;
     std::map<uint32_t, uint32_t> grid_of;  // (traversal << 16 | stack slots) -> persistent grid
     uint32_t tile_w[5] = {8, 8, 8, 8, 8};
-    uint32_t rpl[5] = {1, 1, 1, 1, 1};
     uint32_t grab = 1;
     bool w8 = false;
     bool class_kernels = false;
@@ -328,7 +327,7 @@ bool launch_step(xd_server* s, Step& S, std::string* err) {
         (e = launch_batch(reinterpret_cast<const BatchView*>(S.dev + s->views_off),
                           reinterpret_cast<const uint32_t*>(S.dev + s->tiles_off),
                           reinterpret_cast<const uint32_t*>(S.dev + s->qbeg_off),
-                          reinterpret_cast<uint32_t*>(S.dev + s->counters_off), s->grab, cls, s->w8,
+                          reinterpret_cast<uint32_t*>(S.dev + s->counters_off), pack_grabs(s->grab, nullptr), cls, s->w8,
                           s->mix, S.stream)) != hipSuccess ||
         (e = hipEventRecord(S.ev1, S.stream)) != hipSuccess) {
         *err = std::string("served step launch: ") + hipGetErrorString(e);
@@ -549,10 +548,7 @@ int xd_server_create(xd_context* ctx, const xd_server_config* cfg, xd_server** o
     auto* s = new xd_server();
     s->ctx = ctx;
     s->t0 = std::chrono::steady_clock::now();
-    for (int t = 0; t < 5; ++t) {
-        s->tile_w[t] = c.tuning.tile_w[t];
-        s->rpl[t] = c.tuning.rays_per_lane[t] ? c.tuning.rays_per_lane[t] : 1u;
-    }
+    for (int t = 0; t < 5; ++t) s->tile_w[t] = c.tuning.tile_w[t];
     s->grab = c.tuning.tiles_per_grab;
     s->w8 = c.tuning.waves_per_simd == 8;
     s->class_kernels = c.tuning.class_kernels != 0;
@@ -671,7 +667,9 @@ int xd_server_submit(xd_server* s, const xd_request* req, uint64_t* request_id) 
         u.model_dim[i] = (uint32_t)(vol->kind == XD_VOLUME_GRID ? vol->dims[i] : vol->side);
     }
     u.emission_coeff = r.emission_coeff;
-    make_batch_view(vol, r.traversal, &r.camera, &u, nullptr, r.width, s->tile_w[r.traversal], s->rpl[r.traversal], q->bv,
+    // a request's tiles are queued row by row: the served form takes one ray
+    // per lane per tile (rays_per_lane is read as 1)
+    make_batch_view(vol, r.traversal, &r.camera, &u, nullptr, r.width, s->tile_w[r.traversal], 1u, q->bv,
                     &q->tiles_total);
     if (q->tiles_total > kMaxTilesPerView) {
         delete q;
