@@ -44,6 +44,10 @@ __device__ __forceinline__ uint32_t rope_at(const FastSvo& s, uint32_t leaf, uin
     return *(const __attribute__((address_space(1))) uint32_t*)(s.cw + i);
 }
 
+// The iteration cap as a 32-bit count (a runaway guard that well-formed
+// input never reaches; 32-bit counting saves two instructions per step).
+__device__ __forceinline__ uint32_t cap32_of(uint64_t cap) { return cap > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)cap; }
+
 // A child word names a leaf with zero RGB (whose colour need not be loaded).
 __device__ __forceinline__ bool empty_leaf(const FastSvo& s, uint32_t wd) {
     return (wd & s.empty_tag) || (wd & s.index_mask) >= s.empty_start;
@@ -131,9 +135,10 @@ __device__ __forceinline__ f3 fast_naive(const FastSvo& s, uint64_t cap, f3 ro, 
     t_min = gmax(t_min, 0.0f);
     f3 total = splat(0.0f);
     float t = t_min + min_step;
-    uint64_t it = 0;
+    const uint32_t cap32 = cap32_of(cap);
+    uint32_t it = 0;
     while (t < t_max) {
-        if (++it > cap) break;
+        if (++it > cap32) break;
         f3 pt = add(smul(t, rd), ro);
         f3 offset;
         float side;
@@ -265,7 +270,7 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
     uint4 lo, hi;
     load8(s, node, lo, hi);
     uint32_t mask = child_hits(node_planes(corner, side, rrd, bias));
-    const uint32_t cap32 = cap > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)cap;
+    const uint32_t cap32 = cap32_of(cap);
     uint32_t it = 0;
     for (;;) {
         bool pop = true;
@@ -365,9 +370,10 @@ __device__ __forceinline__ f3 fast_rope(const FastSvo& s, uint64_t cap, f3 ro, f
     node = wd & rope_mask;
     tag = wd;
     offset = add(offset, muls(mul(mask, sgn), side));
-    uint64_t it = 0;
+    const uint32_t cap32 = cap32_of(cap);
+    uint32_t it = 0;
     while (node != 0) {
-        if (++it > cap) break;
+        if (++it > cap32) break;
         pos = add(ro, smul(u_max, rd));
         // find_relative (svo_rope.comp:29-48) from the rope's target
         const uint32_t start = node;
@@ -424,9 +430,10 @@ __device__ __forceinline__ f3 fast_esvo(const FastSvo& s, uint64_t cap, uint32_t
     f3 total = splat(0.0f);
     uint32_t pend_col = 0;  // the last leaf's term, added at the next leaf so that
     float pend_w = 0.0f;    // its colour load overlaps the walk in between
-    uint64_t it = 0;
+    const uint32_t cap32 = cap32_of(cap);
+    uint32_t it = 0;
     while (scale < 23u) {
-        if (++it > cap) break;
+        if (++it > cap32) break;
         f3 t_corner = sub(mul(pos, r.t_coeff), r.t_bias);
         float tc_max = min_elem(t_corner);
         if (t_min <= t_max) {

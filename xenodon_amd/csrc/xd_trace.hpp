@@ -88,10 +88,11 @@ __device__ __forceinline__ f3 dda_trace(const RenderParams& p, const uint32_t* _
     float t = 0.0f;
     f3 total = splat(0.0f);
     const float limit = t_max - t_min;
-    uint64_t it = 0;
+    const uint32_t cap32 = cap32_of(p.cap);
+    uint32_t it = 0;
     if (COUNT) {
         while (t < limit) {
-            if (++it > p.cap) break;
+            if (++it > cap32) break;
             bool mx = sd.x <= gmin(sd.y, sd.z);
             bool my = sd.y <= gmin(sd.z, sd.x);
             bool mz = sd.z <= gmin(sd.x, sd.y);
@@ -126,7 +127,7 @@ __device__ __forceinline__ f3 dda_trace(const RenderParams& p, const uint32_t* _
                 S.addr[j] = 0;
                 S.dts[j] = 0.0f;
                 S.word[j] = 0;
-                if (live && t < limit && ++it <= p.cap) {
+                if (live && t < limit && ++it <= cap32) {
                     bool mx = sd.x <= gmin(sd.y, sd.z);
                     bool my = sd.y <= gmin(sd.z, sd.x);
                     bool mz = sd.z <= gmin(sd.x, sd.y);
@@ -180,7 +181,7 @@ __device__ __forceinline__ f3 dda_trace(const RenderParams& p, const uint32_t* _
             need[j] = false;
             dts[j] = 0.0f;
             addr[j] = 0;
-            if (live && t < limit && ++it <= p.cap) {
+            if (live && t < limit && ++it <= cap32) {
                 bool mx = sd.x <= gmin(sd.y, sd.z);
                 bool my = sd.y <= gmin(sd.z, sd.x);
                 bool mz = sd.z <= gmin(sd.x, sd.y);
