@@ -223,7 +223,11 @@ __device__ __forceinline__ void child_trange(f3 corner, float side, f3 rrd, f3 b
 // Bit k set when child box k of the node is hit (t_min < t_max && t_max > 0,
 // svo_df.comp:25-32).  Per axis the low child spans [T0, T1] and the high
 // child [T1, T2]; min/max of finite floats are exact, so these are the same
-// comparisons as the shader's per-child test.
+// comparisons as the shader's per-child test.  The two comparisons are taken
+// as one, t_max > max(t_min, 0): equal for non-NaN values, and t_min is NaN
+// only when all three of its axis minima are, whose axis pairs then give a
+// NaN t_max too, so both forms are false there.  The 0 joins the y/z partial
+// maxima that the eight children share.
 __device__ __forceinline__ uint32_t child_hits(const Planes& P) {
     const float xa0 = gmin(P.x0, P.x1), xa1 = gmax(P.x0, P.x1), xb0 = gmin(P.x1, P.x2), xb1 = gmax(P.x1, P.x2);
     const float ya0 = gmin(P.y0, P.y1), ya1 = gmax(P.y0, P.y1), yb0 = gmin(P.y1, P.y2), yb1 = gmax(P.y1, P.y2);
@@ -231,9 +235,10 @@ __device__ __forceinline__ uint32_t child_hits(const Planes& P) {
     uint32_t m = 0;
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k) {
-        const float lo = gmax((k & 4u) ? xb0 : xa0, gmax((k & 2u) ? yb0 : ya0, (k & 1u) ? zb0 : za0));
+        const float lo_yz = gmax(gmax((k & 2u) ? yb0 : ya0, (k & 1u) ? zb0 : za0), 0.0f);
+        const float lo = gmax((k & 4u) ? xb0 : xa0, lo_yz);
         const float hi = gmin((k & 4u) ? xb1 : xa1, gmin((k & 2u) ? yb1 : ya1, (k & 1u) ? zb1 : za1));
-        m |= (lo < hi && hi > 0.0f) ? (1u << k) : 0u;
+        m |= (hi > lo) ? (1u << k) : 0u;
     }
     return m;
 }
