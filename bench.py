@@ -111,8 +111,8 @@ def read_cameras(path):
 def host_cpu():
     """The host's CPU model, its logical CPUs, the CPUs this process may run
     on, and the threads the CPU baseline uses: all of those CPUs, or the
-    job's share where OMP_NUM_THREADS sets one (a GPU box allots each GPU
-    job a share of its cores that way)."""
+    job's share where OMP_NUM_THREADS sets one (a shared host allots each
+    GPU job a share of its cores that way)."""
     model = "unknown"
     try:
         for line in open("/proc/cpuinfo"):

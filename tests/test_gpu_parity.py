@@ -3,9 +3,9 @@ restatement of the reference shaders, on the same seeded inputs.
 
 The kernels restate the shaders operation by operation with IEEE single
 precision, so frames are expected to be bit-identical to the oracle.  The
-north-star tolerance against the reference on lavapipe (<= 1 LSB per 8-bit
-channel on >= 99.9 % of pixels, never more than 2 LSB) is asserted as well,
-and the exact-match fraction is reported.
+looser tolerance stated for the shaders on a Vulkan CPU driver (<= 1 LSB per
+8-bit channel on >= 99.9 % of pixels, never more than 2 LSB) is asserted as
+well, and the exact-match fraction is reported.
 """
 import numpy as np
 import pytest
