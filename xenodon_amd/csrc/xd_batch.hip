@@ -38,16 +38,19 @@ __global__ void k_batch_reset(uint32_t* counters) {
 
 // PIPE: the DDA loads the next steps' macro words while it gathers the
 // current ones (more registers; used by the DDA class kernel, whose register
-// budget is its own, not by the all-class kernel).
+// budget is its own, not by the all-class kernel).  For a DDA view,
+// lds_mask says that `lds` holds this wave's copy of the view's macro
+// bitmask: the loop then reads the mask from LDS, as the plain kernel does.
 template <int TRAV, bool PIPE = (TRAV == XD_DDA)>
-__device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds);
+__device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds,
+                                           bool lds_mask = false);
 
 // Any traversal: the kernel for all classes at once dispatches on the view.
 template <>
 __device__ __forceinline__ void shade_tile<kClasses, false>(const BatchView& V, uint32_t tile, uint32_t lane,
-                                                            uint32_t* lds) {
+                                                            uint32_t* lds, bool lds_mask) {
     switch (V.traversal) {
-    case XD_DDA: shade_tile<XD_DDA, false>(V, tile, lane, lds); break;
+    case XD_DDA: shade_tile<XD_DDA, false>(V, tile, lane, lds, lds_mask); break;
     case XD_SVO_NAIVE: shade_tile<XD_SVO_NAIVE>(V, tile, lane, lds); break;
     case XD_ESVO: shade_tile<XD_ESVO>(V, tile, lane, lds); break;
     case XD_SVO_DF: shade_tile<XD_SVO_DF>(V, tile, lane, lds); break;
@@ -56,7 +59,8 @@ __device__ __forceinline__ void shade_tile<kClasses, false>(const BatchView& V, 
 }
 
 template <int TRAV, bool PIPE>
-__device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds) {
+__device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, uint32_t lane, uint32_t* lds,
+                                           bool lds_mask) {
     const uint32_t sh = V.tw_shift;
     const uint32_t tx = tile % V.tiles_x, ty = tile / V.tiles_x;
     const uint32_t ix = (tx << sh) + (lane & ((1u << sh) - 1u));
@@ -71,8 +75,12 @@ __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, ui
         const float side = max_elem(mk((float)p.dims[0], (float)p.dims[1], (float)p.dims[2]));
         const f3 ro = muls(p.cam.translation, side);
         coeff = emission_coeff(p.view, rd) / side;
-        c = p.wide ? dda_trace<false, true, PIPE>(p, p.macro, ro, rd, words)
-                   : dda_trace<false, false, PIPE>(p, p.macro, ro, rd, words);
+        if (lds_mask)
+            c = p.wide ? dda_trace<false, true, false>(p, lds, ro, rd, words)
+                       : dda_trace<false, false, false>(p, lds, ro, rd, words);
+        else
+            c = p.wide ? dda_trace<false, true, PIPE>(p, p.macro, ro, rd, words)
+                       : dda_trace<false, false, PIPE>(p, p.macro, ro, rd, words);
     } else {
         c = p.cw ? svo_pixel<TRAV, false, true>(p, rd, lds, nullptr, words)
                  : svo_pixel<TRAV, false, false>(p, rd, lds, nullptr, words);
@@ -95,6 +103,12 @@ __device__ __forceinline__ void class_body(const BatchView* __restrict__ views, 
     constexpr uint32_t kRanges = TRAV == (int)kClasses ? kClasses * kQueues : kQueues;
     // mixed: consecutive waves of a block start on different classes
     const uint32_t first = mix ? (blockIdx.x / kQueues * 4u + (threadIdx.x >> 6)) % kClasses : 0u;
+    // The macro bitmask of the last DDA volume this wave copied to LDS
+    // (wave-uniform).  Its region is the wave's own part of the block's LDS:
+    // kWaveMacroWords words in the DDA class kernel, the wave's stack region
+    // in the all-class kernel (whose esvo and svo-df tiles overwrite it).  A
+    // mask larger than the region is read from device memory.
+    const uint32_t* staged = nullptr;
     for (uint32_t qi = 0; qi < kRanges; ++qi) {
         // this XCD's range first (of every class, for the all-class kernel),
         // then the next XCD's
@@ -113,7 +127,27 @@ __device__ __forceinline__ void class_body(const BatchView* __restrict__ views, 
             const uint32_t stop = min(end, beg + t + per_grab);
             for (uint32_t i = beg + t; i < stop; ++i) {
                 const uint32_t e = __builtin_amdgcn_readfirstlane(tiles[i]);
-                shade_tile<TRAV>(views[e >> 20], e & (kMaxTilesPerView - 1u), lane, lds);
+                const BatchView& V = views[e >> 20];
+                const uint32_t tile = e & (kMaxTilesPerView - 1u);
+                if (TRAV == XD_DDA || (TRAV == (int)kClasses && V.traversal == XD_DDA)) {
+                    const uint32_t region = TRAV == XD_DDA ? kWaveMacroWords : 2u * V.p.lds_slots * kWaveLanes;
+                    uint32_t* const wave_macro = lds + (threadIdx.x >> 6) * region;
+                    const uint32_t* m = V.p.macro;
+                    const bool fits = V.p.macro_words <= region;
+                    if (fits && m != staged) {
+                        for (uint32_t w = lane; w < V.p.macro_words; w += kWaveLanes) wave_macro[w] = m[w];
+                        // the copy's LDS writes complete before any lane reads
+                        // another lane's words
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_s_waitcnt(0xC07F);
+                        __builtin_amdgcn_wave_barrier();
+                        staged = m;
+                    }
+                    shade_tile<TRAV>(V, tile, lane, wave_macro, fits);
+                } else {
+                    shade_tile<TRAV>(V, tile, lane, lds);
+                    if (TRAV == (int)kClasses && (V.traversal == XD_ESVO || V.traversal == XD_SVO_DF)) staged = nullptr;
+                }
             }
         }
     }
@@ -178,8 +212,12 @@ uint32_t batch_view_slots(const BatchView& B) {
     return (B.traversal == XD_ESVO || B.traversal == XD_SVO_DF) ? B.p.stack_slots : 0u;
 }
 
-size_t batch_lds_bytes(uint32_t slots) {
-    return slots ? (size_t)2 * slots * kBlockThreads * sizeof(uint32_t) : 0;  // 4 waves x own stacks
+size_t batch_lds_bytes(int traversal, uint32_t slots) {
+    if (traversal == XD_DDA) return (size_t)(kBlockThreads / kWaveLanes) * kWaveMacroWords * sizeof(uint32_t);
+    // 4 waves x own stacks of p.lds_slots = max(slots, 1) entries (the
+    // all-class kernel's DDA tiles use that region for their macro bitmask)
+    if (traversal == (int)kClasses) slots = std::max<uint32_t>(slots, 1);
+    return slots ? (size_t)2 * slots * kBlockThreads * sizeof(uint32_t) : 0;
 }
 
 int batch_grid(int device, int traversal, uint32_t slots, bool w8, uint32_t* grid) {
@@ -188,7 +226,7 @@ int batch_grid(int device, int traversal, uint32_t slots, bool w8, uint32_t* gri
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess ||
         (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
                                                           reinterpret_cast<const void*>(kKernels[w8][traversal]),
-                                                          256, batch_lds_bytes(slots))) != hipSuccess)
+                                                          256, batch_lds_bytes(traversal, slots))) != hipSuccess)
         return fail_hip(e, "batch occupancy");
     uint32_t g = (uint32_t)std::max(1, cus * std::max(per_cu, 1));
     *grid = (g + kQueues - 1) / kQueues * kQueues;
@@ -201,7 +239,7 @@ hipError_t launch_batch(const BatchView* views, const uint32_t* tiles, const uin
     for (int c = 0; c <= (int)kClasses; ++c) {
         if (!cls.grid[c]) continue;  // no tiles for this kernel
         const uint32_t base = c < (int)kClasses ? c * kQueues : 0;
-        hipLaunchKernelGGL(kKernels[w8][c], dim3(cls.grid[c]), dim3(256), batch_lds_bytes(cls.slots[c]), s, views,
+        hipLaunchKernelGGL(kKernels[w8][c], dim3(cls.grid[c]), dim3(256), batch_lds_bytes(c, cls.slots[c]), s, views,
                            tiles, qbeg + base, counters + base, grabs, mix ? 1u : 0u);
     }
     return hipGetLastError();

@@ -39,8 +39,13 @@ void make_batch_view(const xd_volume* vol, int traversal, const xd_camera* cam, 
 // Stack slots a view needs in LDS (0 for traversals without a stack).
 uint32_t batch_view_slots(const BatchView& B);
 
-// Dynamic LDS bytes of a class kernel whose deepest stack has `slots` entries.
-size_t batch_lds_bytes(uint32_t slots);
+// Macro-cell bitmasks of at most this many words are staged in LDS by the
+// DDA class kernel, one copy per wave (4 KiB).
+constexpr uint32_t kWaveMacroWords = 1024;
+
+// Dynamic LDS bytes of the persistent kernel of class `traversal` (kClasses:
+// the all-class kernel) whose deepest stack has `slots` entries.
+size_t batch_lds_bytes(int traversal, uint32_t slots);
 
 // Resident blocks of traversal class `traversal`'s persistent kernel (or, for
 // traversal == kClasses, of the all-class kernel) on the whole device (a
