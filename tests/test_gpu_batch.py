@@ -176,3 +176,57 @@ def test_dda_batch_edge_views(env, classes, waves):
     assert lit >= 4
     b.free()
     vol.free()
+
+
+def _macro_words(dims_xyz):
+    """Words of a grid's macro-cell bitmask, from the footprint the library
+    states: bricks of 256 B + 1 occupancy byte, then the bitmask."""
+    nb = int(np.prod([(d + 3) // 4 for d in dims_xyz]))
+    return (abi.volume_footprint(abi.VOLUME_GRID, tuple(dims_xyz)) - nb * 257) // 4
+
+
+@pytest.mark.parametrize("classes,waves", [(1, 8), (1, 0), (0, 8), (0, 0)])
+def test_dda_macro_mask_in_lds_and_in_memory(env, classes, waves):
+    """A batch wave keeps its own LDS copy of a DDA view's macro-cell bitmask
+    when the mask fits its LDS region, and reads it from device memory when it
+    does not.  Views of a grid whose mask exceeds every region (1189 words)
+    alternate with views of two small-mask grids, so waves copy, keep, drop
+    and copy masks again; with one kernel for all traversals, esvo and svo-df
+    tiles overwrite the region between DDA tiles.  Every frame equals the
+    plain render."""
+    ctx, vols = env
+    rng = np.random.default_rng(7)
+    dims = (520, 72, 520)  # x, y, z
+    nb = (dims[2] // 8, dims[1] // 8, dims[0] // 8)
+    cols = rng.integers(0, 256, nb + (4,)).astype(np.uint8)
+    cols[rng.random(nb) < 0.7] = 0
+    cols[..., 3] = 255
+    big = np.ascontiguousarray(np.repeat(np.repeat(np.repeat(cols, 8, 0), 8, 1), 8, 2))
+    assert _macro_words(dims) == 1189 and _macro_words((128, 128, 128)) == 128
+    bvol = ctx.grid_volume(big)
+    md = list(dims)
+    cams = H.orbit_cameras(4, 1.9, seed=11)
+    views = []
+    for k, (f, up, eye) in enumerate(cams):
+        views.append((bvol, abi.DDA, abi.make_camera(f, up, eye), abi.make_uniforms(160, 90, md, emission=2.0),
+                      ctx.frame(160, 90)))
+        small, smd = vols["shells_grid" if k % 2 == 0 else "blocky_grid"]
+        views.append((small, abi.DDA, abi.make_camera(f, up, eye), abi.make_uniforms(96, 72, smd, emission=3.0),
+                      ctx.frame(96, 72)))
+    if classes == 0:
+        svol, smd = vols["shells_svo"]
+        for t in (abi.ESVO, abi.SVO_DF):
+            f, up, eye = cams[1]
+            views.append((svol, t, abi.make_camera(f, up, eye), abi.make_uniforms(128, 96, smd, emission=2.0),
+                          ctx.frame(128, 96)))
+    b = ctx.batch()
+    b.set_tuning([8] * 5, 1, waves, classes, 1, 1)  # views of one size interleaved tile by tile
+    b.set_views(views)
+    for _ in range(2):
+        b.render()
+        _check_against_plain(ctx, views)
+    assert sum(int(v[4].download()[..., :3].any()) for v in views) >= len(views) - 1
+    for v in views:
+        v[4].free()
+    b.free()
+    bvol.free()
