@@ -53,15 +53,17 @@ __device__ __forceinline__ float fsign(float x) { return x > 0.0f ? 1.0f : (x < 
 // GLSL mod(x, y) = x - y * floor(x / y)
 __device__ __forceinline__ float gmod(float x, float y) { return x - y * __builtin_floorf(x / y); }
 
-// c / 255 correctly rounded for c in [0, 255] without a division: the
-// product with fl(1/255) corrected by one exact residual step.  Checked
-// exhaustively against IEEE division for all 256 inputs.
+// c / 255 correctly rounded for c in [0, 255] without a division:
+// fma(c, h, c * l) with h = fl(1/255) and l = fl(1/255 - h).  The sum
+// c*h + fl(c*l) is within about 2^-47 (relative) of c/255, while c/255 lies
+// at least about 2^-33 (relative) from every rounding boundary, so the
+// fma's single rounding lands on the correctly rounded quotient.  Checked
+// exhaustively in exact arithmetic for all 256 inputs.
+constexpr float kInv255Hi = 0x1.010102p-8f;
+constexpr float kInv255Lo = -0x1.fdfdfep-33f;
 __device__ __forceinline__ float unorm8_to_float(uint32_t c) {
-    const float r = 1.0f / 255.0f;
-    float x = (float)c;
-    float q = x * r;
-    float e = __builtin_fmaf(-q, 255.0f, x);
-    return __builtin_fmaf(e, r, q);
+    const float x = (float)c;
+    return __builtin_fmaf(x, kInv255Hi, x * kInv255Lo);
 }
 // unpackUnorm4x8(packed).rgb / texelFetch(RGBA8_UNORM).rgb
 __device__ __forceinline__ f3 unorm_rgb(uint32_t packed) {
