@@ -66,7 +66,7 @@ struct RenderParams {
     uint32_t mn[3];
     uint32_t macro_shift;
     uint32_t macro_words;
-    uint32_t wide;  // padded texel count >= 2^32: 64-bit brick addressing
+    uint32_t wide;  // 64-bit brick addressing: padded texel count >= 2^32, or nbz*nby or nbx >= 2^24
     // octree volume: the .svo words (node_count * 10), used only for trees
     // too large to tag; or the tagged layout: child words + {color, depth}
     const uint32_t* nodes;

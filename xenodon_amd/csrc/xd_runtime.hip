@@ -96,7 +96,12 @@ void fill_params(RenderParams& p, const xd_volume* vol, int traversal, const xd_
         p.macro = vol->d_macro;
         p.macro_shift = vol->macro_shift;
         p.macro_words = vol->macro_words;
-        p.wide = (uint64_t)vol->nb[0] * vol->nb[1] * vol->nb[2] * 64 > 0xFFFFFFFFull ? 1u : 0u;
+        // 32-bit texel indices need the index below 2^32 and, for the 24-bit
+        // multiplies of brick_texel32, nbz * nby and nbx below 2^24
+        p.wide = (uint64_t)vol->nb[0] * vol->nb[1] * vol->nb[2] * 64 > 0xFFFFFFFFull ||
+                         (uint64_t)vol->nb[1] * vol->nb[2] >= (1ull << 24) || vol->nb[0] >= (1u << 24)
+                     ? 1u
+                     : 0u;
         for (int i = 0; i < 3; ++i) {
             p.dims[i] = (uint32_t)vol->dims[i];
             p.nb[i] = vol->nb[i];

@@ -66,6 +66,22 @@ __device__ __forceinline__ int32_t f2i_sat(float x) {
 // trip instead of two in a row.  The steps, the texels fetched and the order
 // of the sum are unchanged.  Without PIPE the bitmask may be in LDS or in
 // device memory.
+// Bit of voxel (ux, uy, uz)'s macro cell in the bitmask.  A bitmask holds at
+// most kMaxMacroWords * 32 = 2^17 cells, so every operand and partial result
+// is below 2^24 and the 24-bit multiply-add (full rate) is exact.
+__device__ __forceinline__ uint32_t macro_bit(uint32_t ux, uint32_t uy, uint32_t uz, uint32_t ms, uint32_t mnx,
+                                              uint32_t mny) {
+    return __umul24(__umul24(uz >> ms, mny) + (uy >> ms), mnx) + (ux >> ms);
+}
+
+// Texel index of voxel (ux, uy, uz) in the bricked grid, 32-bit form.  Grids
+// take it only when nbz * nby < 2^24 and nbx < 2^24 (else the 64-bit form;
+// see fill_params), so both multiplies' operands are below 2^24.
+__device__ __forceinline__ uint32_t brick_texel32(uint32_t ux, uint32_t uy, uint32_t uz, uint32_t nbx, uint32_t nby,
+                                                  uint32_t in_brick) {
+    return ((__umul24(__umul24(uz >> 2, nby) + (uy >> 2), nbx) + (ux >> 2)) << 6) + in_brick;
+}
+
 template <bool COUNT, bool WIDE, bool PIPE = false>
 __device__ __forceinline__ f3 dda_trace(const RenderParams& p, const uint32_t* __restrict__ lds_macro, f3 ro, f3 rd,
                                         uint64_t& words) {
@@ -134,14 +150,14 @@ __device__ __forceinline__ f3 dda_trace(const RenderParams& p, const uint32_t* _
                     float t0 = min_elem(sd);
                     if ((uint32_t)px < X && (uint32_t)py < Y && (uint32_t)pz < Z) {
                         const uint32_t ux = (uint32_t)px, uy = (uint32_t)py, uz = (uint32_t)pz;
-                        const uint32_t bit = ((uz >> ms) * mny + (uy >> ms)) * mnx + (ux >> ms);
+                        const uint32_t bit = macro_bit(ux, uy, uz, ms, mnx, mny);
                         S.word[j] = gmacro[bit >> 5];
                         S.bits |= (bit & 31u) << (5 * j);
                         const uint32_t in_brick = (ux & 3u) + ((uy & 3u) << 2) + ((uz & 3u) << 4);
                         if (WIDE)
                             S.addr[j] = (((uint64_t)(uz >> 2) * nby + (uy >> 2)) * nbx + (ux >> 2)) * 64u + in_brick;
                         else
-                            S.addr[j] = (uint32_t)(((((uz >> 2) * nby + (uy >> 2)) * nbx + (ux >> 2)) << 6) + in_brick);
+                            S.addr[j] = brick_texel32(ux, uy, uz, nbx, nby, in_brick);
                     }
                     S.dts[j] = t0 - t;
                     t = t0;
@@ -188,14 +204,14 @@ __device__ __forceinline__ f3 dda_trace(const RenderParams& p, const uint32_t* _
                 float t0 = min_elem(sd);
                 if ((uint32_t)px < X && (uint32_t)py < Y && (uint32_t)pz < Z) {
                     const uint32_t ux = (uint32_t)px, uy = (uint32_t)py, uz = (uint32_t)pz;
-                    const uint32_t bit = ((uz >> ms) * mny + (uy >> ms)) * mnx + (ux >> ms);
+                    const uint32_t bit = macro_bit(ux, uy, uz, ms, mnx, mny);
                     if ((lds_macro[bit >> 5] >> (bit & 31u)) & 1u) {
                         need[j] = true;
                         const uint32_t in_brick = (ux & 3u) + ((uy & 3u) << 2) + ((uz & 3u) << 4);
                         if (WIDE)
                             addr[j] = (((uint64_t)(uz >> 2) * nby + (uy >> 2)) * nbx + (ux >> 2)) * 64u + in_brick;
                         else
-                            addr[j] = (uint32_t)(((((uz >> 2) * nby + (uy >> 2)) * nbx + (ux >> 2)) << 6) + in_brick);
+                            addr[j] = brick_texel32(ux, uy, uz, nbx, nby, in_brick);
                     }
                 }
                 dts[j] = t0 - t;
