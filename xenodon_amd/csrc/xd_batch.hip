@@ -82,8 +82,8 @@ __device__ __forceinline__ void shade_tile(const BatchView& V, uint32_t tile, ui
             c = p.wide ? dda_trace<false, true, PIPE>(p, p.macro, ro, rd, words)
                        : dda_trace<false, false, PIPE>(p, p.macro, ro, rd, words);
     } else {
-        c = p.cw ? svo_pixel<TRAV, false, true>(p, rd, lds, nullptr, words)
-                 : svo_pixel<TRAV, false, false>(p, rd, lds, nullptr, words);
+        c = p.cw ? svo_pixel<TRAV, false, true, false>(p, rd, lds, nullptr, words)
+                 : svo_pixel<TRAV, false, false, false>(p, rd, lds, nullptr, words);
         coeff = emission_coeff(p.view, rd);
     }
     p.out[(uint64_t)iy * p.out_pitch + ix] = pack_rgba(muls(c, coeff));

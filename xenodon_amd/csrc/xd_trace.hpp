@@ -239,7 +239,11 @@ __device__ __forceinline__ f3 dda_trace(const RenderParams& p, const uint32_t* _
 //   TAGGED && !COUNT  fast traversals on the tagged layout (xd_svo_fast.hpp)
 //   otherwise         reference-structured traversals (xd_svo_ref.hpp)
 // ---------------------------------------------------------------------------
-template <int TRAV, bool COUNT, bool TAGGED>
+// LDS_TOP: the kernel may stage top-level child words in LDS (plain
+// launches; only svo-rope's views do).  Where no view can, the traversal is
+// built with no top levels at all, which drops the LDS-or-device-memory
+// choice from every child-word read.
+template <int TRAV, bool COUNT, bool TAGGED, bool LDS_TOP = true>
 __device__ __forceinline__ f3 svo_pixel(const RenderParams& p, f3 rd, uint32_t* lds, const uint32_t* lds_top,
                                         uint64_t& words) {
     // this wavefront's stack region: two arrays of lds_slots x 64 lane-strided
@@ -248,7 +252,8 @@ __device__ __forceinline__ f3 svo_pixel(const RenderParams& p, f3 rd, uint32_t* 
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     uint32_t* stk_a = lds + wave * 2u * p.lds_slots * kWaveLanes + lane;
     uint32_t* stk_b = stk_a + p.lds_slots * kWaveLanes;
-    const FastSvo fs{p.cw, p.meta, p.root_tag, lds_top, p.top_nodes, p.node_count, p.depth_exact, p.table, p.rope_depth,
+    const uint32_t n_top = LDS_TOP && TRAV == 4 ? p.top_nodes : 0u;  // svo-rope
+    const FastSvo fs{p.cw, p.meta, p.root_tag, lds_top, n_top, p.node_count, p.depth_exact, p.table, p.rope_depth,
                       p.index_mask, p.empty_tag, p.empty_start, p.leaf_base};
     if (TRAV == 2) {  // esvo
         float* stk_tmax = reinterpret_cast<float*>(stk_b);
