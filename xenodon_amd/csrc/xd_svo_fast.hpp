@@ -262,6 +262,7 @@ __device__ __forceinline__ uint32_t child_hits(const Planes& P) {
 // they do not, a popped node can be placed elsewhere than where it was
 // entered, so the walk follows the shader literally: every non-last child
 // pushes, and a popped node's remaining children are tested again.
+template <bool EXACT>
 __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t slots, f3 ro, f3 rd,
                                       uint32_t* stk_node, uint32_t* stk_child) {
     f3 rrd = divv(splat(1.0f), rd);
@@ -293,7 +294,7 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
                     total = add(total, muls(unorm_rgb(s.meta[wd & s.index_mask]), t_max - gmax(t_min, 0.0f)));
                 }
                 if (mask != 0) continue;  // next sibling, from registers
-            } else if (!(push = s.depth_exact ? mask != 0 : ci != 7u) || sp < cap_sp) {
+            } else if (!(push = EXACT ? mask != 0 : ci != 7u) || sp < cap_sp) {
                 if (push) {
                     stk_node[sp * kWaveLanes] = node;
                     stk_child[sp * kWaveLanes] = mask | (ci << 8) | (depth << 11);
@@ -316,7 +317,7 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
             const uint32_t e = stk_child[sp * kWaveLanes];
             mask = e & 0xFFu;
             depth = (e >> 11) & 0xFFu;
-            const uint32_t d = s.depth_exact ? depth : (s.meta[(uint64_t)s.n + node] & kDepthMask);
+            const uint32_t d = EXACT ? depth : (s.meta[(uint64_t)s.n + node] & kDepthMask);
             const f3 pos = mk(corner.x + side, corner.y + side, corner.z + side);
             side = __builtin_ldexpf(1.0f, -(int)d) * 0.5f;
             // side is a power of two, so mod(pos, 2 side) is exact with the reciprocal
@@ -327,7 +328,7 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
         load8(s, node, lo, hi);
         if (!pop) {
             mask = child_hits(node_planes(corner, side, rrd, bias));
-        } else if (!s.depth_exact) {
+        } else if (!EXACT) {
             // the shader tests the remaining children at the geometry the
             // depth field gives the popped node, which may differ from the
             // geometry they were first tested at

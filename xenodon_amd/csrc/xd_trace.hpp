@@ -273,8 +273,9 @@ __device__ __forceinline__ f3 svo_pixel(const RenderParams& p, f3 rd, uint32_t* 
     }
     const f3 ro = p.cam.translation;
     if (TRAV == 3) {  // svo-df
-        if (TAGGED && !COUNT) return fast_df(fs, p.cap, p.stack_slots, ro, rd, stk_a,
-                                             stk_b);
+        if (TAGGED && !COUNT)  // depth-field handling fixed per tree (a uniform choice)
+            return p.depth_exact ? fast_df<true>(fs, p.cap, p.stack_slots, ro, rd, stk_a, stk_b)
+                                 : fast_df<false>(fs, p.cap, p.stack_slots, ro, rd, stk_a, stk_b);
         if (TAGGED) return ref_df<COUNT>(TaggedNodes{p.cw, p.meta, p.node_count, p.index_mask, p.leaf_base}, p.cap, p.stack_slots, ro, rd, stk_a, stk_b, words);
         return ref_df<COUNT>(RawNodes{p.nodes, p.node_count}, p.cap, p.stack_slots, ro, rd, stk_a, stk_b, words);
     }
