@@ -3,8 +3,9 @@
 // Layout: child_w[n*8 + k] = index of child k | kTagLeaf if that child is a
 // leaf | kTagEmpty if it is a leaf whose RGB is zero; meta[n] = {color,
 // is_leaf_depth}.  A leaf's child words are its ropes, tagged the same way.
-// Trees above 2^30 nodes carry 31-bit indices and no empty tag (index_mask
-// and empty_tag of FastSvo say which).
+// Trees above 2^30 nodes carry 31-bit indices and no empty tag; their empty
+// leaves are the indices from a threshold on (empty_mask and empty_min of
+// FastSvo cover both).
 // The traversal arithmetic is the shaders' (see xd_svo_ref.hpp); only loads
 // change:
 //   * "is this node a leaf?" comes with the pointer that led to it, so a
@@ -30,8 +31,11 @@ struct FastSvo {
     const uint2* table;    // start table: kTableCells entries {child word, level}, or null
     uint32_t rope_depth;   // ropes carry their targets' depth fields (kRopeDepthShift)
     uint32_t index_mask;   // index bits of a child word
-    uint32_t empty_tag;    // kTagEmpty, or 0 where leaves carry no empty tag
-    uint32_t empty_start;  // ... where they do not, the empty leaves' first index
+    // a child word wd names an empty leaf iff (wd & empty_mask) >= empty_min:
+    // {kTagEmpty, kTagEmpty} where leaves carry the empty tag, else {index
+    // mask, first empty leaf index} (see make_fast_svo)
+    uint32_t empty_mask;
+    uint32_t empty_min;
     uint32_t leaf_base;    // first leaf: the leaves' ropes are stored as six planes
 };
 
@@ -49,8 +53,18 @@ __device__ __forceinline__ uint32_t rope_at(const FastSvo& s, uint32_t leaf, uin
 __device__ __forceinline__ uint32_t cap32_of(uint64_t cap) { return cap > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)cap; }
 
 // A child word names a leaf with zero RGB (whose colour need not be loaded).
-__device__ __forceinline__ bool empty_leaf(const FastSvo& s, uint32_t wd) {
-    return (wd & s.empty_tag) || (wd & s.index_mask) >= s.empty_start;
+__device__ __forceinline__ bool empty_leaf(const FastSvo& s, uint32_t wd) { return (wd & s.empty_mask) >= s.empty_min; }
+
+// FastSvo of a tagged tree.  empty_tag = kTagEmpty marks empty leaves by
+// that bit (wd & kTagEmpty is 0 or kTagEmpty, and those layouts set
+// empty_start to 0xFFFFFFFF, which no index reaches); empty_tag = 0 by the
+// index range from empty_start on.  One comparison covers both.
+__device__ __forceinline__ FastSvo make_fast_svo(const uint32_t* cw, const uint32_t* meta, uint32_t root_tag,
+                                                 const uint32_t* top, uint32_t n_top, uint32_t n, uint32_t depth_exact,
+                                                 const uint2* table, uint32_t rope_depth, uint32_t index_mask,
+                                                 uint32_t empty_tag, uint32_t empty_start, uint32_t leaf_base) {
+    return FastSvo{cw, meta, root_tag, top, n_top, n, depth_exact, table, rope_depth, index_mask,
+                   empty_tag ? empty_tag : index_mask, empty_tag ? empty_tag : empty_start, leaf_base};
 }
 
 // Child word k of node n: from LDS for the breadth-first top levels.

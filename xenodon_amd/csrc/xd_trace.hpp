@@ -253,8 +253,8 @@ __device__ __forceinline__ f3 svo_pixel(const RenderParams& p, f3 rd, uint32_t* 
     uint32_t* stk_a = lds + wave * 2u * p.lds_slots * kWaveLanes + lane;
     uint32_t* stk_b = stk_a + p.lds_slots * kWaveLanes;
     const uint32_t n_top = LDS_TOP && TRAV == 4 ? p.top_nodes : 0u;  // svo-rope
-    const FastSvo fs{p.cw, p.meta, p.root_tag, lds_top, n_top, p.node_count, p.depth_exact, p.table, p.rope_depth,
-                      p.index_mask, p.empty_tag, p.empty_start, p.leaf_base};
+    const FastSvo fs = make_fast_svo(p.cw, p.meta, p.root_tag, lds_top, n_top, p.node_count, p.depth_exact, p.table, p.rope_depth,
+                      p.index_mask, p.empty_tag, p.empty_start, p.leaf_base);
     if (TRAV == 2) {  // esvo
         float* stk_tmax = reinterpret_cast<float*>(stk_b);
         for (uint32_t s = 0; s < p.stack_slots; ++s) {
