@@ -334,10 +334,21 @@ __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t s
             const uint32_t d = EXACT ? depth : (s.meta[(uint64_t)s.n + node] & kDepthMask);
             const f3 pos = mk(corner.x + side, corner.y + side, corner.z + side);
             side = __builtin_ldexpf(1.0f, -(int)d) * 0.5f;
-            // side is a power of two, so mod(pos, 2 side) is exact with the reciprocal
-            const float s2 = side * 2.0f;
-            const float inv = __builtin_ldexpf(1.0f, -__builtin_amdgcn_frexp_expf(s2) + 1);
-            corner = sub(pos, mk(pmod(pos.x, s2, inv), pmod(pos.y, s2, inv), pmod(pos.z, s2, inv)));
+            if (EXACT) {
+                // The tree's geometry: pos and every corner are multiples of
+                // 2^-23 in [0, 1] (at most 23 levels), and s2 = 2^-d.  Then
+                // pos * 2^d, its floor times s2 and both subtractions of
+                // pos - mod(pos, s2) are exact, so that corner is the
+                // product s2 * floor(pos * 2^d).
+                const float s2 = side * 2.0f, inv = __builtin_ldexpf(1.0f, (int)d);
+                corner = mk(s2 * __builtin_floorf(pos.x * inv), s2 * __builtin_floorf(pos.y * inv),
+                            s2 * __builtin_floorf(pos.z * inv));
+            } else {
+                // side is a power of two, so mod(pos, 2 side) is exact with the reciprocal
+                const float s2 = side * 2.0f;
+                const float inv = __builtin_ldexpf(1.0f, -__builtin_amdgcn_frexp_expf(s2) + 1);
+                corner = sub(pos, mk(pmod(pos.x, s2, inv), pmod(pos.y, s2, inv), pmod(pos.z, s2, inv)));
+            }
         }
         load8(s, node, lo, hi);
         if (!pop) {
