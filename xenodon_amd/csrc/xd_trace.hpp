@@ -192,37 +192,35 @@ __device__ __forceinline__ f3 dda_trace(const RenderParams& p, const uint32_t* _
         typename std::conditional<WIDE, uint64_t, uint32_t>::type addr[B];
         float dts[B];
         bool need[B];
+        // the steps as selects rather than branches: a step that does not run
+        // (the ray ended, or the iteration cap) leaves the state as it is
+        // (x + 0 = x for every side distance, which are +0 or more); its
+        // bitmask read goes to word 0 and its texel is not fetched
 #pragma unroll
         for (int j = 0; j < B; ++j) {
-            need[j] = false;
-            dts[j] = 0.0f;
-            addr[j] = 0;
-            if (live && t < limit && ++it <= cap32) {
-                bool mx = sd.x <= gmin(sd.y, sd.z);
-                bool my = sd.y <= gmin(sd.z, sd.x);
-                bool mz = sd.z <= gmin(sd.x, sd.y);
-                float t0 = min_elem(sd);
-                if ((uint32_t)px < X && (uint32_t)py < Y && (uint32_t)pz < Z) {
-                    const uint32_t ux = (uint32_t)px, uy = (uint32_t)py, uz = (uint32_t)pz;
-                    const uint32_t bit = macro_bit(ux, uy, uz, ms, mnx, mny);
-                    if ((lds_macro[bit >> 5] >> (bit & 31u)) & 1u) {
-                        need[j] = true;
-                        const uint32_t in_brick = (ux & 3u) + ((uy & 3u) << 2) + ((uz & 3u) << 4);
-                        if (WIDE)
-                            addr[j] = (((uint64_t)(uz >> 2) * nby + (uy >> 2)) * nbx + (ux >> 2)) * 64u + in_brick;
-                        else
-                            addr[j] = brick_texel32(ux, uy, uz, nbx, nby, in_brick);
-                    }
-                }
-                dts[j] = t0 - t;
-                t = t0;
-                sd = add(sd, mk(mx ? t_delta.x : 0.0f, my ? t_delta.y : 0.0f, mz ? t_delta.z : 0.0f));
-                px += mx ? sx : 0;
-                py += my ? sy : 0;
-                pz += mz ? sz : 0;
-            } else {
-                live = false;
-            }
+            const bool go = live && t < limit;
+            it += go ? 1u : 0u;
+            const bool act = go && it <= cap32;
+            live = act;
+            const bool mx = act && sd.x <= gmin(sd.y, sd.z);
+            const bool my = act && sd.y <= gmin(sd.z, sd.x);
+            const bool mz = act && sd.z <= gmin(sd.x, sd.y);
+            const float t0 = min_elem(sd);
+            const uint32_t ux = (uint32_t)px, uy = (uint32_t)py, uz = (uint32_t)pz;
+            const bool inside = act && ux < X && uy < Y && uz < Z;
+            const uint32_t bit = inside ? macro_bit(ux, uy, uz, ms, mnx, mny) : 0u;
+            need[j] = inside && ((lds_macro[bit >> 5] >> (bit & 31u)) & 1u);
+            const uint32_t in_brick = (ux & 3u) + ((uy & 3u) << 2) + ((uz & 3u) << 4);
+            if (WIDE)
+                addr[j] = (((uint64_t)(uz >> 2) * nby + (uy >> 2)) * nbx + (ux >> 2)) * 64u + in_brick;
+            else
+                addr[j] = brick_texel32(ux, uy, uz, nbx, nby, in_brick);
+            dts[j] = act ? t0 - t : 0.0f;
+            t = act ? t0 : t;
+            sd = add(sd, mk(mx ? t_delta.x : 0.0f, my ? t_delta.y : 0.0f, mz ? t_delta.z : 0.0f));
+            px += mx ? sx : 0;
+            py += my ? sy : 0;
+            pz += mz ? sz : 0;
         }
         uint32_t tex[B];
 #pragma unroll
