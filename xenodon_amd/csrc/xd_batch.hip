@@ -137,7 +137,8 @@ __device__ __forceinline__ void class_body(const BatchView* __restrict__ views, 
                     if (fits && m != staged) {
                         for (uint32_t w = lane; w < V.p.macro_words; w += kWaveLanes) wave_macro[w] = m[w];
                         // the copy's LDS writes complete before any lane reads
-                        // another lane's words
+                        // another lane's words: s_waitcnt lgkmcnt(0) (0xC07F
+                        // leaves the vmcnt and expcnt fields at their maxima)
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                         __builtin_amdgcn_s_waitcnt(0xC07F);
                         __builtin_amdgcn_wave_barrier();
