@@ -311,3 +311,53 @@ def test_dag_and_single_leaf_trees_bit_exact(ctx, t):
             img, _ = ctx.render_image(gv, t, abi.make_camera(f, up, eye), abi.make_uniforms(*args, emission=1.5))
             ref = xo.render(t, ov, xo.make_camera(f, up, eye), xo.make_uniforms(*args, emission=1.5))["rgba"]
             _assert_parity(img, ref, f"{name}/{abi.TRAVERSAL_NAMES[t]}/cam{k}")
+
+
+def _chain_tree(depth, target, seed):
+    """A tree whose every level splits once more towards `target`: at each
+    depth the node holds seven leaves (some with zero RGB) and one child that
+    continues, down to a coloured leaf at `depth`.  Node words as the .svo
+    format has them: 8 children, RGBA colour, leaf bit | depth."""
+    rng = np.random.default_rng(seed)
+    rows = [[0] * 10]
+    cur = 0
+    for d in range(depth):
+        k = sum((int(target[a] * (1 << (d + 1))) & 1) << (2 - a) for a in range(3))
+        for c in range(8):
+            rows.append([0] * 10)
+            idx = len(rows) - 1
+            rows[cur][c] = idx
+            if c == k and d + 1 < depth:
+                rows[idx][9] = d + 1  # internal, exact depth field
+                nxt = idx
+            else:
+                rgb = 0 if rng.random() < 0.3 else int(rng.integers(1, 1 << 24))
+                rows[idx][8] = 0xFF000000 | rgb
+                rows[idx][9] = 0x80000000 | (d + 1)
+        if d + 1 < depth:
+            cur = nxt
+    return np.array(rows, np.uint32)
+
+
+@pytest.mark.parametrize("t", [abi.SVO_NAIVE, abi.ESVO, abi.SVO_DF, abi.SVO_ROPE])
+def test_deep_tree_bit_exact(ctx, t):
+    """A tree 22 levels deep (side 2^22; the device accepts up to 23), seen
+    from an eye inside its deepest cells: every descent, pop and rope step
+    down to the finest level, where the corners and sides are the smallest
+    dyadic values of the walk, renders as the oracle does."""
+    depth, target = 22, (0.3712345, 0.6123457, 0.5317891)
+    nodes = _chain_tree(depth, target, seed=5)
+    side = 1 << depth
+    if t == abi.SVO_ROPE:
+        xo.lib().xo_generate_ropes(nodes.ctypes.data, nodes.shape[0], side)
+    gv = ctx.svo_volume(nodes, side)
+    assert gv.info().depth_fields_exact == 1
+    ov = xo.OracleVolume(nodes=nodes, side=side)
+    eyes = [np.array(target) + np.array(o) for o in [(3e-7, 2e-7, -4e-7), (-2e-6, 1e-6, 3e-6), (0.4, 0.3, -0.5)]]
+    for k, eye in enumerate(eyes):
+        f, up, e = H.look_at(tuple(eye), target)
+        args = (64, 48, [side] * 3)
+        img, _ = ctx.render_image(gv, t, abi.make_camera(f, up, e), abi.make_uniforms(*args, emission=2.0))
+        ref = xo.render(t, ov, xo.make_camera(f, up, e), xo.make_uniforms(*args, emission=2.0))["rgba"]
+        _assert_parity(img, ref, f"deep/{abi.TRAVERSAL_NAMES[t]}/eye{k}")
+        assert ref[..., :3].any()
