@@ -275,7 +275,9 @@ __device__ __forceinline__ uint32_t child_hits(const Planes& P) {
 // depth in the tree, the depth kept on the stack is that same value.  When
 // they do not, a popped node can be placed elsewhere than where it was
 // entered, so the walk follows the shader literally: every non-last child
-// pushes, and a popped node's remaining children are tested again.
+// pushes, and a popped node's remaining children are tested again.  EXACT
+// selects the first walk (the tree's depth fields are exact, s.depth_exact);
+// each walk is built on its own so that neither carries the other's code.
 template <bool EXACT>
 __device__ __forceinline__ f3 fast_df(const FastSvo& s, uint64_t cap, uint32_t slots, f3 ro, f3 rd,
                                       uint32_t* stk_node, uint32_t* stk_child) {
